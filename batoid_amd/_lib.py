@@ -1,0 +1,182 @@
+"""ctypes binding of ``libbatoid_amd.so`` (declared in include/batoid_amd.h).
+
+This is the thin host side of the C ABI: the POD descriptor structs, the
+loader, and helpers that pack descriptor records into device buffers.  The
+library is mandatory: if it is missing, or no GPU is present, every compute
+call raises; nothing falls back to CPU arithmetic.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libbatoid_amd.so")
+
+c_double_p = ctypes.POINTER(ctypes.c_double)
+c_u8_p = ctypes.POINTER(ctypes.c_uint8)
+
+
+class BtRays(ctypes.Structure):
+    _fields_ = [(name, ctypes.c_void_p) for name in (
+        "x", "y", "z", "vx", "vy", "vz", "t", "wavelength", "flux",
+        "vignetted", "failed")] + [("n", ctypes.c_int64)]
+
+
+class BtSurface(ctypes.Structure):
+    _fields_ = [
+        ("kind", ctypes.c_int32), ("nchild", ctypes.c_int32),
+        ("n1", ctypes.c_int32), ("n2", ctypes.c_int32),
+        ("flags", ctypes.c_int32), ("pad_", ctypes.c_int32),
+        ("p", ctypes.c_double * 12),
+        ("a0", ctypes.c_void_p), ("a1", ctypes.c_void_p),
+        ("a2", ctypes.c_void_p), ("a3", ctypes.c_void_p),
+    ]
+
+
+class BtMedium(ctypes.Structure):
+    _fields_ = [
+        ("kind", ctypes.c_int32), ("n", ctypes.c_int32),
+        ("p", ctypes.c_double * 6),
+        ("a0", ctypes.c_void_p), ("a1", ctypes.c_void_p),
+    ]
+
+
+class BtObsc(ctypes.Structure):
+    _fields_ = [
+        ("op", ctypes.c_int32), ("n", ctypes.c_int32),
+        ("p", ctypes.c_double * 8),
+        ("a0", ctypes.c_void_p), ("a1", ctypes.c_void_p),
+    ]
+
+
+class BtStep(ctypes.Structure):
+    _fields_ = [
+        ("action", ctypes.c_int32), ("surface", ctypes.c_int32),
+        ("medium", ctypes.c_int32), ("obsc", ctypes.c_int32),
+        ("obsc_len", ctypes.c_int32), ("pad_", ctypes.c_int32),
+        ("dr", ctypes.c_double * 3), ("drot", ctypes.c_double * 9),
+    ]
+
+
+class BtPlan(ctypes.Structure):
+    _fields_ = [
+        ("nstep", ctypes.c_int32), ("flags", ctypes.c_int32),
+        ("steps", ctypes.c_void_p), ("surfaces", ctypes.c_void_p),
+        ("media", ctypes.c_void_p), ("obscs", ctypes.c_void_p),
+    ]
+
+
+STRUCTS = (BtRays, BtSurface, BtMedium, BtObsc, BtStep, BtPlan)
+
+# enum values (include/batoid_amd.h)
+PLANE, SPHERE, PARABOLOID, QUADRIC, ASPHERE, TILTED, BICUBIC, POLYNOMIAL, SUM = range(9)
+MED_CONST, MED_TABLE, MED_SELLMEIER, MED_SUMITA, MED_AIR = range(5)
+(OB_CIRCLE, OB_ANNULUS, OB_RECTANGLE, OB_RAY, OB_POLYGON, OB_NEGATION,
+ OB_UNION, OB_INTERSECTION) = range(8)
+ACT_INTERSECT, ACT_REFLECT, ACT_REFRACT = range(3)
+PLAN_NO_REFRACT = 1
+PSF_FP32, PSF_BF16X3 = 0, 1
+OBSC_MAX_STACK = 32
+MAX_SUM_CHILDREN = 8
+
+# name -> (restype, argtypes)
+_V = ctypes.c_void_p
+_D = ctypes.c_double
+_I = ctypes.c_int
+_L = ctypes.c_int64
+SIGNATURES = {
+    "bt_abi_version": (_I, []),
+    "bt_last_error": (ctypes.c_char_p, []),
+    "bt_struct_sizes": (_I, [ctypes.POINTER(ctypes.c_int64), _I]),
+    "bt_device_info": (_I, [ctypes.POINTER(_I)] * 3),
+    "bt_apply_forward_transform": (_I, [_V, _V, BtRays, _V]),
+    "bt_apply_reverse_transform": (_I, [_V, _V, BtRays, _V]),
+    "bt_apply_forward_transform_arrays": (_I, [_V, _V, _V, _V, _V, _L, _V]),
+    "bt_apply_reverse_transform_arrays": (_I, [_V, _V, _V, _V, _V, _L, _V]),
+    "bt_finish_parallel": (_I, [_V, _V, _V, _V, _V, _V, _L, _V]),
+    "bt_intersect": (_I, [_V, _V, _V, BtRays, _I, _V, _I, _V]),
+    "bt_reflect": (_I, [_V, _V, _V, BtRays, _I, _V, _I, _V]),
+    "bt_refract": (_I, [_V, _V, _V, _V, BtRays, _I, _V, _I, _V]),
+    "bt_obscure": (_I, [_V, _I, BtRays, _V]),
+    "bt_trace_plan": (_I, [BtPlan, BtRays, BtRays, _I, _V]),
+    "bt_surface_sag": (_I, [_V, _V, _V, _V, _L, _V]),
+    "bt_surface_normal": (_I, [_V, _V, _V, _V, _L, _V]),
+    "bt_medium_get_n": (_I, [_V, _V, _V, _L, _V]),
+    "bt_obsc_contains": (_I, [_V, _I, _V, _V, _V, _L, _V]),
+    "bt_position_at_time": (_I, [BtRays, _D, _V, _V, _V, _V]),
+    "bt_propagate_in_place": (_I, [BtRays, _D, _V]),
+    "bt_phase": (_I, [BtRays, _D, _D, _D, _D, _V, _V]),
+    "bt_amplitude": (_I, [BtRays, _D, _D, _D, _D, _V, _V]),
+    "bt_sum_amplitude_workspace_bytes": (_L, [_L]),
+    "bt_sum_amplitude": (_I, [BtRays, _D, _D, _D, _D, _I, _V, _V, _V]),
+    "bt_huygens_workspace_bytes": (_L, [_L, _I, _I]),
+    "bt_huygens_direct": (_I, [BtRays, _V, _V, _V, _I, _I, _D, _I, _V, _V, _L, _V]),
+    "bt_huygens_gemm": (_I, [BtRays, _V, _V, _V, _I, _I, _D, _I, _I, _V, _V, _L, _V]),
+}
+
+
+class BatoidAMDError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load(path=LIB_PATH):
+    """Load the HIP library (once) and declare every C-ABI signature."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise BatoidAMDError(
+            f"{path} is missing: build it with `python -m batoid_amd.build`"
+        )
+    lib = ctypes.CDLL(path)
+    for name, (restype, argtypes) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = restype
+        fn.argtypes = argtypes
+    sizes = (ctypes.c_int64 * len(STRUCTS))()
+    lib.bt_struct_sizes(sizes, len(STRUCTS))
+    for st, sz in zip(STRUCTS, sizes):
+        if ctypes.sizeof(st) != sz:
+            raise BatoidAMDError(
+                f"ABI mismatch: sizeof({st.__name__}) is {sz} in the library, "
+                f"{ctypes.sizeof(st)} in the binding"
+            )
+    _lib = lib
+    return lib
+
+
+def check(rc, what=""):
+    if rc != 0:
+        msg = load().bt_last_error()
+        msg = msg.decode() if msg else ""
+        raise BatoidAMDError(f"{what} failed with status {rc}: {msg}")
+
+
+def call(name, *args):
+    """Call a C-ABI function and raise on a nonzero status."""
+    rc = getattr(load(), name)(*args)
+    check(rc, name)
+    return rc
+
+
+def dvec(values):
+    """A host double array argument (dr, drot, ...)."""
+    arr = (ctypes.c_double * len(values))(*[float(v) for v in values])
+    return arr
+
+
+def struct_array_bytes(records):
+    """Raw bytes of a ctypes struct sequence, for upload."""
+    if not records:
+        return b""
+    typ = type(records[0])
+    arr = (typ * len(records))(*records)
+    return bytes(memoryview(arr).cast("B"))
+
+
+def np_ptr(a):
+    return a.ctypes.data if a is not None else None

@@ -1,0 +1,660 @@
+// bt_device.h -- per-ray FP64 device functions of the trace hot path.
+//
+// Surface sag / normal / time-to-intersect, media n(lambda) and obscuration
+// programs, evaluated on descriptors that live in device memory.  All
+// threads of a launch read the same descriptor, so its fields are
+// wave-uniform (scalar loads); only the per-ray arithmetic is vector work.
+//
+// Operation order follows the reference expression by expression and the
+// library is compiled with -ffp-contract=off, so with IEEE-correct FP64
+// division and square root the results are bit-identical to the reference's
+// CPU build (which has no FMA on baseline x86-64).  Citations are to
+// jmeyers314/batoid.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "../../include/batoid_amd.h"
+
+#define BT_DEV __device__ __forceinline__
+
+namespace bt {
+
+// ---------------------------------------------------------------------------
+// Bicubic table (src/table.cpp:42-158)
+// ---------------------------------------------------------------------------
+BT_DEV double spline1(double x, double v0, double v1, double d0, double d1)
+{
+    double a = 2*(v0-v1) + d0 + d1;
+    double b = 3*(v1-v0) - 2*d0 - d1;
+    double c = d0;
+    double d = v0;
+    return d + x*(c + x*(b + x*a));
+}
+
+BT_DEV double grad1(double x, double v0, double v1, double d0, double d1)
+{
+    double a = 2*(v0-v1) + d0 + d1;
+    double b = 3*(v1-v0) - 2*d0 - d1;
+    double c = d0;
+    return c + x*(2*b + x*3*a);
+}
+
+// int(std::floor(u)) as baseline x86-64 evaluates it: NaN and values outside
+// the int range become INT_MIN and fail the cell range test.  (The GPU's own
+// conversion would map NaN to 0, a valid cell.)
+BT_DEV int cell_index(double u)
+{
+    double f = floor(u);
+    if (!(f >= -2147483648.0 && f < 2147483648.0)) return (int)0x80000000;
+    return (int)f;
+}
+
+BT_DEV bool table_cell(const bt_surface* s, double x, double y, int& ix, int& iy)
+{
+    ix = cell_index((x - s->p[0])/s->p[2]);
+    iy = cell_index((y - s->p[1])/s->p[3]);
+    return !(ix >= s->n1-1 || ix < 0 || iy >= s->n2-1 || iy < 0);
+}
+
+BT_DEV double table_eval(const bt_surface* s, double x, double y)
+{
+    int ix, iy;
+    if (!table_cell(s, x, y, ix, iy))
+        return s->flags ? __builtin_nan("") : 0.0;
+    const double dx = s->p[2], dy = s->p[3];
+    const double* __restrict__ z = s->a0;
+    const double* __restrict__ zx = s->a1;
+    const double* __restrict__ zy = s->a2;
+    const double* __restrict__ zxy = s->a3;
+    double xgrid = s->p[0] + ix*dx;
+    double ygrid = s->p[1] + iy*dy;
+    double xfrac = (x - xgrid)/dx;
+    double yfrac = (y - ygrid)/dy;
+    const int64_t k0 = (int64_t)s->n1*iy + ix, k1 = k0 + s->n1;
+    double val0 = spline1(xfrac, z[k0], z[k0+1], zx[k0]*dx, zx[k0+1]*dx);
+    double val1 = spline1(xfrac, z[k1], z[k1+1], zx[k1]*dx, zx[k1+1]*dx);
+    double der0 = spline1(xfrac, zy[k0], zy[k0+1], zxy[k0]*dx, zxy[k0+1]*dx);
+    double der1 = spline1(xfrac, zy[k1], zy[k1+1], zxy[k1]*dx, zxy[k1+1]*dx);
+    return spline1(yfrac, val0, val1, der0*dy, der1*dy);
+}
+
+BT_DEV void table_grad(const bt_surface* s, double x, double y,
+                       double& dzdx, double& dzdy)
+{
+    int ix, iy;
+    if (!table_cell(s, x, y, ix, iy)) {
+        double v = s->flags ? __builtin_nan("") : 0.0;
+        dzdx = v; dzdy = v;
+        return;
+    }
+    const double dx = s->p[2], dy = s->p[3];
+    const double* __restrict__ z = s->a0;
+    const double* __restrict__ zx = s->a1;
+    const double* __restrict__ zy = s->a2;
+    const double* __restrict__ zxy = s->a3;
+    double xgrid = s->p[0] + ix*dx;
+    double ygrid = s->p[1] + iy*dy;
+    double xfrac = (x - xgrid)/dx;
+    double yfrac = (y - ygrid)/dy;
+    const int64_t k0 = (int64_t)s->n1*iy + ix, k1 = k0 + s->n1;
+    // the 16 table values of the cell, each read once
+    const double z00 = z[k0], z01 = z[k0+1], z10 = z[k1], z11 = z[k1+1];
+    const double x00 = zx[k0], x01 = zx[k0+1], x10 = zx[k1], x11 = zx[k1+1];
+    const double y00 = zy[k0], y01 = zy[k0+1], y10 = zy[k1], y11 = zy[k1+1];
+    const double c00 = zxy[k0], c01 = zxy[k0+1], c10 = zxy[k1], c11 = zxy[k1+1];
+
+    double val0 = grad1(xfrac, z00, z01, x00*dx, x01*dx);
+    double val1 = grad1(xfrac, z10, z11, x10*dx, x11*dx);
+    double der0 = grad1(xfrac, y00, y01, c00*dx, c01*dx);
+    double der1 = grad1(xfrac, y10, y11, c10*dx, c11*dx);
+    dzdx = spline1(yfrac, val0, val1, der0*dy, der1*dy)/dx;
+
+    val0 = grad1(yfrac, z00, z10, y00*dy, y10*dy);
+    val1 = grad1(yfrac, z01, z11, y01*dy, y11*dy);
+    der0 = grad1(yfrac, x00, x10, c00*dy, c10*dy);
+    der1 = grad1(yfrac, x01, x11, c01*dy, c11*dy);
+    dzdy = spline1(xfrac, val0, val1, der0*dx, der1*dx)/dy;
+}
+
+// ---------------------------------------------------------------------------
+// 2-D Horner (src/polynomialSurface.cpp:50-66)
+// ---------------------------------------------------------------------------
+BT_DEV double horner2d(double x, double y, const double* __restrict__ c,
+                       int nx, int ny)
+{
+    double r = 0.0;
+    for (int i = ny-1; i >= 0; i--) {
+        const double* row = c + (int64_t)i*nx;
+        double h = 0.0;
+        for (int j = nx-1; j >= 0; j--) { h *= y; h += row[j]; }
+        r *= x;
+        r += h;
+    }
+    return r;
+}
+
+// ---------------------------------------------------------------------------
+// Quadric family (src/quadric.cpp, src/asphere.cpp)
+// ---------------------------------------------------------------------------
+BT_DEV double quadric_sag(const bt_surface* s, double x, double y)
+{
+    const double R = s->p[0];
+    double r2 = x*x + y*y;
+    if (R != 0)
+        return r2/(R*(1.+sqrt(1.-r2*s->p[8])));
+    return 0.0;
+}
+
+BT_DEV double quadric_dzdr(const bt_surface* s, double r)
+{
+    const double R = s->p[0];
+    if (R != 0.0)
+        return r/(R*sqrt(1-r*r*s->p[8]));
+    return 0.0;
+}
+
+BT_DEV double asphere_dzdr(const bt_surface* s, double r)
+{
+    double result = quadric_dzdr(s, r);
+    double rr = r*r;
+    double rrr = r;
+    const double* __restrict__ dc = s->a1;
+    const int n = s->n1;
+    for (int i = 0; i < n; i++) {
+        result += dc[i]*rrr;
+        rrr *= rr;
+    }
+    return result;
+}
+
+// ---------------------------------------------------------------------------
+// Leaf surfaces: sag and normal
+// ---------------------------------------------------------------------------
+BT_DEV double leaf_sag(const bt_surface* s, double x, double y)
+{
+    switch (s->kind) {
+    case BT_PLANE:                                   // plane.cpp:15-17
+        return 0.0;
+    case BT_SPHERE:                                  // sphere.cpp:17-21
+        if (s->p[0] != 0)
+            return s->p[0]*(1-sqrt(1-(x*x + y*y)*s->p[3]));
+        return 0.0;
+    case BT_PARABOLOID:                              // paraboloid.cpp:17-23
+        if (s->p[0] != 0) {
+            double r2 = x*x + y*y;
+            return r2*s->p[2];
+        }
+        return 0.0;
+    case BT_QUADRIC:                                 // quadric.cpp:18-25
+        return quadric_sag(s, x, y);
+    case BT_ASPHERE: {                               // asphere.cpp:37-46
+        double r2 = x*x + y*y;
+        double rr = 1.0;
+        double result = quadric_sag(s, x, y);
+        const double* __restrict__ c = s->a0;
+        const int n = s->n1;
+        for (int i = 0; i < n; i++) {
+            rr *= r2;
+            result += c[i]*rr;
+        }
+        return result;
+    }
+    case BT_TILTED:                                  // tilted.cpp:16-18
+        return x*s->p[0] + y*s->p[1];
+    case BT_BICUBIC:                                 // bicubic.cpp:19-21
+        return table_eval(s, x, y);
+    case BT_POLYNOMIAL:                              // polynomialSurface.cpp:35-37
+        return horner2d(x-s->p[0], y-s->p[1], s->a0, s->n1, s->n2);
+    }
+    return __builtin_nan("");
+}
+
+BT_DEV void leaf_normal(const bt_surface* s, double x, double y,
+                        double& nx, double& ny, double& nz)
+{
+    switch (s->kind) {
+    case BT_PLANE:                                   // plane.cpp:19-23
+        nx = 0.0; ny = 0.0; nz = 1.0;
+        return;
+    case BT_SPHERE: {                                // sphere.cpp:23-35, 66-69
+        double rsqr = x*x + y*y;
+        if (s->p[0] == 0.0 || rsqr == 0.0) {
+            nx = 0.0; ny = 0.0; nz = 1.0;
+        } else {
+            double r = sqrt(rsqr);
+            double rat = r*s->p[2];
+            double dzdr = rat/sqrt(1-rat*rat);
+            nz = 1/sqrt(1+dzdr*dzdr);
+            nx = -x/r*dzdr*nz;
+            ny = -y/r*dzdr*nz;
+        }
+        return;
+    }
+    case BT_PARABOLOID: {                            // paraboloid.cpp:25-37
+        if (s->p[0] == 0) {
+            nx = 0.0; ny = 0.0; nz = 1.0;
+        } else {
+            const double Rinv = s->p[1];
+            nz = 1/sqrt(1+(x*x+y*y)*Rinv*Rinv);
+            nx = -x*Rinv*nz;
+            ny = -y*Rinv*nz;
+        }
+        return;
+    }
+    case BT_QUADRIC:                                 // quadric.cpp:27-39
+    case BT_ASPHERE: {                               // asphere.cpp:48-62
+        double r = sqrt(x*x + y*y);
+        if (r == 0.0) {
+            nx = 0.0; ny = 0.0; nz = 1.0;
+        } else {
+            double dzdr = (s->kind == BT_QUADRIC) ? quadric_dzdr(s, r)
+                                                  : asphere_dzdr(s, r);
+            nz = 1/sqrt(1+dzdr*dzdr);
+            nx = -x/r*dzdr*nz;
+            ny = -y/r*dzdr*nz;
+        }
+        return;
+    }
+    case BT_TILTED: {                                // tilted.cpp:20-27
+        const double tx = s->p[0], ty = s->p[1];
+        nx = -tx;
+        ny = -ty;
+        nz = sqrt(1 - tx*tx - ty*ty);
+        return;
+    }
+    case BT_BICUBIC: {                               // bicubic.cpp:23-40
+        double dxdz, dydz;
+        table_grad(s, x, y, dxdz, dydz);
+        if (__builtin_isnan(dxdz)) {
+            nx = __builtin_nan(""); ny = nx; nz = nx;
+            return;
+        }
+        nz = 1/sqrt(1 + dxdz*dxdz + dydz*dydz);
+        nx = -dxdz*nz;
+        ny = -dydz*nz;
+        return;
+    }
+    case BT_POLYNOMIAL: {                            // polynomialSurface.cpp:39-48
+        double gx = -horner2d(x-s->p[0], y-s->p[1], s->a1, s->n1-1, s->n2-1);
+        double gy = -horner2d(x-s->p[0], y-s->p[1], s->a2, s->n1-1, s->n2-1);
+        nz = 1./sqrt(gx*gx + gy*gy + 1);
+        nx = gx*nz;
+        ny = gy*nz;
+        return;
+    }
+    }
+    nx = __builtin_nan(""); ny = nx; nz = nx;
+}
+
+// Sum records are followed by their children (sum.cpp:23-45).
+BT_DEV double surf_sag(const bt_surface* s, double x, double y)
+{
+    if (s->kind != BT_SUM) return leaf_sag(s, x, y);
+    double result = 0.0;
+    const int nc = s->nchild;
+    for (int i = 0; i < nc; i++)
+        result += leaf_sag(s + 1 + i, x, y);
+    return result;
+}
+
+BT_DEV void surf_normal(const bt_surface* s, double x, double y,
+                        double& nx, double& ny, double& nz)
+{
+    if (s->kind != BT_SUM) { leaf_normal(s, x, y, nx, ny, nz); return; }
+    double sx = 0.0, sy = 0.0;
+    const int nc = s->nchild;
+    for (int i = 0; i < nc; i++) {
+        double tnx, tny, tnz;
+        leaf_normal(s + 1 + i, x, y, tnx, tny, tnz);
+        sx += tnx/tnz;
+        sy += tny/tnz;
+    }
+    nz = 1./sqrt(sx*sx + sy*sy + 1);
+    nx = sx*nz;
+    ny = sy*nz;
+}
+
+// Tangent-plane Newton with exactly `niter` iterations (surface.cpp:22-49).
+BT_DEV bool newton(const bt_surface* s, double x, double y, double z,
+                   double vx, double vy, double vz, double& dt, int niter)
+{
+    double rPx = x+vx*dt;
+    double rPy = y+vy*dt;
+    double rPz = z+vz*dt;
+    double sz = surf_sag(s, rPx, rPy);
+    for (int iter = 0; iter < niter; iter++) {
+        double nx, ny, nz;
+        surf_normal(s, rPx, rPy, nx, ny, nz);
+        dt = (rPx-x)*nx + (rPy-y)*ny + (sz-z)*nz;
+        dt /= (nx*vx + ny*vy + nz*vz);
+        rPx = x+vx*dt;
+        rPy = y+vy*dt;
+        rPz = z+vz*dt;
+        sz = surf_sag(s, rPx, rPy);
+    }
+    return fabs(sz-rPz) < 1e-12;
+}
+
+// std::max(a, b) = (a < b) ? b : a; std::min(a, b) = (b < a) ? b : a.
+BT_DEV double pick_root(double vzR, double dt1, double dt2)
+{
+    if (vzR < 0) return (dt1 < dt2) ? dt2 : dt1;
+    return (dt2 < dt1) ? dt2 : dt1;
+}
+
+BT_DEV bool quadric_tti(const bt_surface* s, double x, double y, double z,
+                        double vx, double vy, double vz, double& dt)
+{                                                    // quadric.cpp:41-70
+    double z0term = z-s->p[6];
+    double vrr0 = vx*x + vy*y;
+    double a = vz*vz + (vx*vx+vy*vy)*s->p[5];
+    double b = 2*(vz*z0term + vrr0*s->p[5]);
+    double c = z0term*z0term - s->p[7] + (x*x + y*y)*s->p[5];
+    double discriminant = b*b - 4*a*c;
+    if (discriminant < 0)
+        return false;
+    double dt1;
+    if (b > 0)
+        dt1 = (-b - sqrt(discriminant)) / (2*a);
+    else
+        dt1 = 2*c / (-b + sqrt(discriminant));
+    double dt2 = c / (a*dt1);
+    double z1 = z + vz*dt1;
+    double z2 = z + vz*dt2;
+    dt = (fabs(z1) < fabs(z2)) ? dt1 : dt2;
+    return true;
+}
+
+BT_DEV bool leaf_tti(const bt_surface* s, double x, double y, double z,
+                     double vx, double vy, double vz, double& dt, int niter)
+{
+    switch (s->kind) {
+    case BT_PLANE:                                   // plane.cpp:25-31
+        if (vz == 0) return false;
+        dt = -z/vz;
+        return true;
+    case BT_SPHERE: {                                // sphere.cpp:37-64
+        const double R = s->p[0];
+        double vr2 = vx*vx + vy*vy;
+        double vz2 = vz*vz;
+        double vrr0 = vx*x + vy*y;
+        double r02 = x*x + y*y;
+        double z0term = z-R;
+        double a = vz2 + vr2;
+        double b = 2*vz*z0term + 2*vrr0;
+        double c = z0term*z0term - s->p[1] + r02;
+        double discriminant = b*b - 4*a*c;
+        if (discriminant < 0) return false;
+        double dt1;
+        if (b > 0)
+            dt1 = (-b - sqrt(discriminant)) / (2*a);
+        else
+            dt1 = 2*c / (-b + sqrt(discriminant));
+        double dt2 = c / (a*dt1);
+        dt = pick_root(vz*R, dt1, dt2);
+        return true;
+    }
+    case BT_PARABOLOID: {                            // paraboloid.cpp:39-71
+        double a = (vx*vx + vy*vy)*s->p[2];
+        double b = (x*vx + y*vy)*s->p[1] - vz;
+        double c = (x*x + y*y)*s->p[2] - z;
+        if (a == 0) {
+            dt = -c/b;
+            return true;
+        }
+        double discriminant = b*b - 4*a*c;
+        if (discriminant < 0) return false;
+        double dt1;
+        if (b > 0)
+            dt1 = (-b - sqrt(discriminant)) / (2*a);
+        else
+            dt1 = 2*c / (-b + sqrt(discriminant));
+        double dt2 = c / (a*dt1);
+        dt = pick_root(vz*s->p[0], dt1, dt2);
+        return true;
+    }
+    case BT_QUADRIC:
+        return quadric_tti(s, x, y, z, vx, vy, vz, dt);
+    case BT_ASPHERE:                                 // asphere.cpp:75-85
+        if (!quadric_tti(s, x, y, z, vx, vy, vz, dt)) return false;
+        return newton(s, x, y, z, vx, vy, vz, dt, niter);
+    case BT_TILTED: {                                // tilted.cpp:29-37
+        double num = z - x*s->p[0] - y*s->p[1];
+        double den = vx*s->p[0] + vy*s->p[1] - vz;
+        if (den == 0) return false;
+        dt = num/den;
+        return true;
+    }
+    case BT_BICUBIC:
+    case BT_POLYNOMIAL:                              // surface.cpp:22-49
+        return newton(s, x, y, z, vx, vy, vz, dt, niter);
+    }
+    return false;
+}
+
+BT_DEV bool surf_tti(const bt_surface* s, double x, double y, double z,
+                     double vx, double vy, double vz, double& dt, int niter)
+{
+    if (s->kind != BT_SUM) return leaf_tti(s, x, y, z, vx, vy, vz, dt, niter);
+    // sum.cpp:47-56: first child's intersection seeds Newton on the Sum
+    if (!leaf_tti(s + 1, x, y, z, vx, vy, vz, dt, niter)) return false;
+    return newton(s, x, y, z, vx, vy, vz, dt, niter);
+}
+
+// ---------------------------------------------------------------------------
+// Media (src/medium.cpp)
+// ---------------------------------------------------------------------------
+BT_DEV double medium_n(const bt_medium* m, double w)
+{
+    switch (m->kind) {
+    case BT_MED_CONST:                               // medium.cpp:30-32
+        return m->p[0];
+    case BT_MED_TABLE: {                             // medium.cpp:89-105
+        const double* __restrict__ args = m->a0;
+        const double* __restrict__ vals = m->a1;
+        const int size = m->n;
+        if (w < args[0]) return __builtin_nan("");
+        if (w > args[size-1]) return __builtin_nan("");
+        int u;
+        for (u = 1; u < size; u++)
+            if (w < args[u]) break;
+        double out = (w - args[u-1]);
+        out *= (vals[u] - vals[u-1]);
+        out /= (args[u] - args[u-1]);
+        out += vals[u-1];
+        return out;
+    }
+    case BT_MED_SELLMEIER: {                         // medium.cpp:145-149
+        const double* p = m->p;
+        double x = w*w*1e12;
+        return sqrt(1.0 + p[0]*x/(x-p[3]) + p[1]*x/(x-p[4]) + p[2]*x/(x-p[5]));
+    }
+    case BT_MED_SUMITA: {                            // medium.cpp:186-191
+        const double* A = m->p;
+        double x = w*w*1e12;
+        double y = 1./x;
+        return sqrt(A[0] + A[1]*x + y*(A[2] + y*(A[3] + y*(A[4] + y*A[5]))));
+    }
+    case BT_MED_AIR: {                               // medium.cpp:237-245
+        const double P = m->p[0], T = m->p[1], W = m->p[2];
+        double s2 = 1e-12 / (w*w);
+        double n_minus_one = (64.328 + (29498.1 / (146.0 - s2))
+                              + (255.4 / (41.0 - s2))) * 1.e-6;
+        n_minus_one *= P * (1.0 + (1.049 - 0.0157 * T) * 1.e-6 * P)
+                       / (720.883 * (1.0 + 0.003661 * T));
+        n_minus_one -= (0.0624 - 0.000680 * s2)/(1.0 + 0.003661 * T) * W * 1.e-6;
+        return 1+n_minus_one;
+    }
+    }
+    return __builtin_nan("");
+}
+
+// ---------------------------------------------------------------------------
+// Obscuration programs (src/obscuration.cpp); the stack is a bit mask.
+// ---------------------------------------------------------------------------
+BT_DEV bool polygon_contains(const bt_obsc* o, double x, double y)
+{                                                    // obscuration.cpp:201-225
+    const double* __restrict__ xp = o->a0;
+    const double* __restrict__ yp = o->a1;
+    const int size = o->n;
+    double x1 = xp[0];
+    double y1 = yp[0];
+    double xinters = 0.0;
+    bool inside = false;
+    for (int i = 1; i <= size; i++) {
+        const int k = (i == size) ? 0 : i;
+        double x2 = xp[k];
+        double y2 = yp[k];
+        double ymin = (y2 < y1) ? y2 : y1;
+        double ymax = (y1 < y2) ? y2 : y1;
+        double xmax = (x1 < x2) ? x2 : x1;
+        if (y > ymin && y <= ymax && x <= xmax) {
+            if (y1 != y2)
+                xinters = (y-y1)*(x2-x1)/(y2-y1)+x1;
+            if (x1 == x2 || x <= xinters)
+                inside = !inside;
+        }
+        x1 = x2;
+        y1 = y2;
+    }
+    return inside;
+}
+
+BT_DEV bool obsc_contains(const bt_obsc* prog, int len, double x, double y)
+{
+    uint32_t stack = 0;   // bit k = stack slot k
+    int sp = 0;
+    for (int k = 0; k < len; k++) {
+        const bt_obsc* o = prog + k;
+        const double* p = o->p;
+        bool v;
+        switch (o->op) {
+        case BT_OB_CIRCLE:                           // obscuration.cpp:32-34
+            v = hypot(x-p[1], y-p[2]) < p[0];
+            break;
+        case BT_OB_ANNULUS: {                        // obscuration.cpp:80-83
+            double h = hypot(x-p[2], y-p[3]);
+            v = (p[0] <= h) && (h < p[1]);
+            break;
+        }
+        case BT_OB_RECTANGLE: {                      // obscuration.cpp:117-121
+            double xp = (x-p[2])*p[4] + (y-p[3])*p[5];
+            double yp = -(x-p[2])*p[5] + (y-p[3])*p[4];
+            v = (xp > -p[0]/2 && xp < p[0]/2 && yp > -p[1]/2 && yp < p[1]/2);
+            break;
+        }
+        case BT_OB_RAY: {                            // obscuration.cpp:155-159
+            double xp = (x-p[1])*p[3] + (y-p[2])*p[4];
+            double yp = -(x-p[1])*p[4] + (y-p[2])*p[3];
+            v = (xp > 0.0 && yp > -p[0]/2 && yp < p[0]/2);
+            break;
+        }
+        case BT_OB_POLYGON:
+            v = polygon_contains(o, x, y);
+            break;
+        case BT_OB_NEGATION:                         // obscuration.cpp:320-322
+            stack ^= (1u << (sp-1));
+            continue;
+        case BT_OB_UNION:                            // obscuration.cpp:364-370
+        case BT_OB_INTERSECTION: {                   // obscuration.cpp:415-421
+            const int m = o->n;
+            const uint32_t mask = ((m >= 32) ? 0xffffffffu : ((1u << m) - 1u)) << (sp - m);
+            const uint32_t bits = stack & mask;
+            v = (o->op == BT_OB_UNION) ? (bits != 0) : (bits == mask);
+            stack &= ~mask;
+            sp -= m;
+            break;
+        }
+        default:
+            v = false;
+        }
+        stack = (stack & ~(1u << sp)) | ((uint32_t)v << sp);
+        sp++;
+    }
+    return sp > 0 && ((stack >> (sp-1)) & 1u);
+}
+
+// ---------------------------------------------------------------------------
+// One Interface: transform, intersect, interact, obscure
+// (src/batoid.cpp:221-505 and optic.py:419-422)
+// ---------------------------------------------------------------------------
+struct Ray {
+    double x, y, z, vx, vy, vz, t, w;
+    bool vig, fail;
+};
+
+// Returns true when the ray's kinematic state was (re)written.
+BT_DEV bool apply_interaction(int action, const double* __restrict__ dr,
+                              const double* __restrict__ drot,
+                              const bt_surface* surf, const bt_medium* m2,
+                              Ray& r, int niter)
+{
+    if (r.fail) return false;
+    double dx = r.x-dr[0];
+    double dy = r.y-dr[1];
+    double dz = r.z-dr[2];
+    double x = dx*drot[0] + dy*drot[3] + dz*drot[6];
+    double y = dx*drot[1] + dy*drot[4] + dz*drot[7];
+    double z = dx*drot[2] + dy*drot[5] + dz*drot[8];
+    double vx = r.vx*drot[0] + r.vy*drot[3] + r.vz*drot[6];
+    double vy = r.vx*drot[1] + r.vy*drot[4] + r.vz*drot[7];
+    double vz = r.vx*drot[2] + r.vy*drot[5] + r.vz*drot[8];
+    double t = r.t;
+    double dt = 0.0;
+    if (!surf_tti(surf, x, y, z, vx, vy, vz, dt, niter)) {
+        r.fail = true;
+        r.vig = true;
+        return false;          // left in the previous frame
+    }
+    x += vx * dt;
+    y += vy * dt;
+    z += vz * dt;
+    t += dt;
+    if (action == BT_ACT_REFLECT) {                  // batoid.cpp:340-353
+        double nx, ny, nz;
+        surf_normal(surf, x, y, nx, ny, nz);
+        double alpha = vx*nx;
+        alpha += vy*ny;
+        alpha += vz*nz;
+        vx -= 2*alpha*nx;
+        vy -= 2*alpha*ny;
+        vz -= 2*alpha*nz;
+    } else if (action == BT_ACT_REFRACT) {           // batoid.cpp:453-496
+        double n1 = vx*vx;
+        n1 += vy*vy;
+        n1 += vz*vz;
+        n1 = 1/sqrt(n1);
+        double nvx = vx*n1;
+        double nvy = vy*n1;
+        double nvz = vz*n1;
+        double nx, ny, nz;
+        surf_normal(surf, x, y, nx, ny, nz);
+        double alpha = nvx*nx;
+        alpha += nvy*ny;
+        alpha += nvz*nz;
+        if (alpha > 0.) {
+            nx *= -1; ny *= -1; nz *= -1;
+            alpha *= -1;
+        }
+        double n2 = medium_n(m2, r.w);
+        double eta = n1/n2;
+        double sinsqr = eta*eta*(1-alpha*alpha);
+        if (sinsqr > 1.0) {                          // total internal reflection
+            vx = __builtin_nan(""); vy = vx; vz = vx;
+            r.fail = true;
+            r.vig = true;
+        } else {
+            double nfactor = eta*alpha + sqrt(1-sinsqr);
+            vx = (eta*nvx - nfactor*nx) / n2;
+            vy = (eta*nvy - nfactor*ny) / n2;
+            vz = (eta*nvz - nfactor*nz) / n2;
+        }
+    }
+    r.x = x; r.y = y; r.z = z;
+    r.vx = vx; r.vy = vy; r.vz = vz;
+    r.t = t;
+    return true;
+}
+
+}  // namespace bt

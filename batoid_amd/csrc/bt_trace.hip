@@ -1,0 +1,462 @@
+// bt_trace.hip -- ray-surface trace kernels and their C-ABI entry points.
+//
+// Data layout: the RayVector is 11 structure-of-arrays columns in HBM
+// (9 x FP64 + 2 x u8 flags); lane i of a wave handles ray i, so every column
+// access is a fully coalesced 512 B (FP64) or 64 B (flag) wave transaction.
+// Descriptors (surfaces, media, obscuration programs, plan steps) are read
+// through wave-uniform addresses and land in scalar registers.
+//
+// Two ways to trace:
+//   * per-surface kernels (bt_intersect/bt_reflect/bt_refract), one HBM read
+//     + write of the touched columns per Interface, with the Interface's
+//     obscuration folded into the same pass;
+//   * the fused plan kernel (bt_trace_plan), which applies every Interface of
+//     an optic while the ray sits in registers: one read and one write per
+//     ray for the whole optic.
+#include <hip/hip_runtime.h>
+#include <string>
+#include <cstdio>
+
+#include "bt_device.h"
+#include "bt_common.h"
+
+namespace bt {
+
+thread_local std::string g_last_error;
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+int fail(int code, const std::string& msg)
+{
+    set_error(msg);
+    return code;
+}
+
+int check_launch(const char* what)
+{
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        set_error(std::string(what) + ": " + hipGetErrorString(e));
+        return (int)e;
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Kernels
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int64_t ray_index()
+{
+    return (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+}
+
+template <bool kForward>
+__global__ __launch_bounds__(kBlock) void k_transform_rays(Xform xf, bt_rays rv)
+{                                                    // batoid.cpp:71-148
+    const int64_t i = ray_index();
+    if (i >= rv.n) return;
+    const double* dr = xf.dr;
+    const double* m = xf.drot;
+    const double x0 = rv.x[i], y0 = rv.y[i], z0 = rv.z[i];
+    const double vx0 = rv.vx[i], vy0 = rv.vy[i], vz0 = rv.vz[i];
+    if (kForward) {
+        double dx = x0-dr[0];
+        double dy = y0-dr[1];
+        double dz = z0-dr[2];
+        rv.x[i] = dx*m[0] + dy*m[3] + dz*m[6];
+        rv.y[i] = dx*m[1] + dy*m[4] + dz*m[7];
+        rv.z[i] = dx*m[2] + dy*m[5] + dz*m[8];
+        rv.vx[i] = vx0*m[0] + vy0*m[3] + vz0*m[6];
+        rv.vy[i] = vx0*m[1] + vy0*m[4] + vz0*m[7];
+        rv.vz[i] = vx0*m[2] + vy0*m[5] + vz0*m[8];
+    } else {
+        rv.x[i] = x0*m[0] + y0*m[1] + z0*m[2] + dr[0];
+        rv.y[i] = x0*m[3] + y0*m[4] + z0*m[5] + dr[1];
+        rv.z[i] = x0*m[6] + y0*m[7] + z0*m[8] + dr[2];
+        rv.vx[i] = vx0*m[0] + vy0*m[1] + vz0*m[2];
+        rv.vy[i] = vx0*m[3] + vy0*m[4] + vz0*m[5];
+        rv.vz[i] = vx0*m[6] + vy0*m[7] + vz0*m[8];
+    }
+}
+
+template <bool kForward>
+__global__ __launch_bounds__(kBlock) void k_transform_arrays(
+    Xform xf, double* __restrict__ x, double* __restrict__ y,
+    double* __restrict__ z, int64_t n)
+{                                                    // batoid.cpp:8-38
+    const int64_t i = ray_index();
+    if (i >= n) return;
+    const double* dr = xf.dr;
+    const double* m = xf.drot;
+    if (kForward) {
+        double dx = x[i]-dr[0];
+        double dy = y[i]-dr[1];
+        double dz = z[i]-dr[2];
+        x[i] = dx*m[0] + dy*m[3] + dz*m[6];
+        y[i] = dx*m[1] + dy*m[4] + dz*m[7];
+        z[i] = dx*m[2] + dy*m[5] + dz*m[8];
+    } else {
+        const double x0 = x[i], y0 = y[i], z0 = z[i];
+        x[i] = x0*m[0] + y0*m[1] + z0*m[2] + dr[0];
+        y[i] = x0*m[3] + y0*m[4] + z0*m[5] + dr[1];
+        z[i] = x0*m[6] + y0*m[7] + z0*m[8] + dr[2];
+    }
+}
+
+struct FinishArgs {
+    Xform xf;
+    double vl[3];   // -vv rotated into the stop frame (batoid.cpp:44-46)
+};
+
+__global__ __launch_bounds__(kBlock) void k_finish_parallel(
+    FinishArgs a, double* __restrict__ x, double* __restrict__ y,
+    double* __restrict__ z, int64_t n)
+{                                                    // batoid.cpp:40-69
+    const int64_t i = ray_index();
+    if (i >= n) return;
+    const double* dr = a.xf.dr;
+    const double* m = a.xf.drot;
+    double dx = x[i]-dr[0];
+    double dy = y[i]-dr[1];
+    double dz = z[i]-dr[2];
+    double xx = dx*m[0] + dy*m[3] + dz*m[6];
+    double yy = dx*m[1] + dy*m[4] + dz*m[7];
+    double zz = dx*m[2] + dy*m[5] + dz*m[8];
+    double dt = -zz/a.vl[2];
+    xx += dt*a.vl[0];
+    yy += dt*a.vl[1];
+    zz += dt*a.vl[2];
+    x[i] = xx*m[0] + yy*m[1] + zz*m[2] + dr[0];
+    y[i] = xx*m[3] + yy*m[4] + zz*m[5] + dr[1];
+    z[i] = xx*m[6] + yy*m[7] + zz*m[8] + dr[2];
+}
+
+// One Interface over all rays, in place.
+__global__ __launch_bounds__(kBlock) void k_step(
+    int action, Xform xf, const bt_surface* __restrict__ surf,
+    const bt_medium* __restrict__ m2, const bt_obsc* __restrict__ obsc,
+    int obsc_len, bt_rays rv, int niter)
+{
+    const int64_t i = ray_index();
+    if (i >= rv.n) return;
+    Ray r;
+    r.fail = rv.failed[i] != 0;
+    r.vig = rv.vignetted[i] != 0;
+    const bool fail0 = r.fail, vig0 = r.vig;
+    r.x = rv.x[i]; r.y = rv.y[i]; r.z = rv.z[i];
+    r.vx = rv.vx[i]; r.vy = rv.vy[i]; r.vz = rv.vz[i];
+    r.t = rv.t[i];
+    r.w = (action == BT_ACT_REFRACT && !r.fail) ? rv.wavelength[i] : 0.0;
+    const bool moved = apply_interaction(action, xf.dr, xf.drot, surf, m2, r, niter);
+    if (moved) {
+        rv.x[i] = r.x; rv.y[i] = r.y; rv.z[i] = r.z;
+        rv.vx[i] = r.vx; rv.vy[i] = r.vy; rv.vz[i] = r.vz;
+        rv.t[i] = r.t;
+    }
+    if (obsc_len > 0 && !r.vig)                     // batoid.cpp:169-171
+        r.vig = obsc_contains(obsc, obsc_len, r.x, r.y);
+    if (r.vig != vig0) rv.vignetted[i] = r.vig;
+    if (r.fail != fail0) rv.failed[i] = r.fail;
+}
+
+__global__ __launch_bounds__(kBlock) void k_obscure(
+    const bt_obsc* __restrict__ obsc, int obsc_len, bt_rays rv)
+{                                                    // batoid.cpp:151-172
+    const int64_t i = ray_index();
+    if (i >= rv.n) return;
+    if (rv.vignetted[i]) return;
+    if (obsc_contains(obsc, obsc_len, rv.x[i], rv.y[i])) rv.vignetted[i] = 1;
+}
+
+// Every step of a plan applied while the ray stays in registers.
+__global__ __launch_bounds__(kBlock) void k_plan(
+    bt_plan plan, bt_rays in, bt_rays out, int niter, int need_w)
+{
+    const int64_t i = ray_index();
+    if (i >= in.n) return;
+    Ray r;
+    r.fail = in.failed[i] != 0;
+    r.vig = in.vignetted[i] != 0;
+    r.x = in.x[i]; r.y = in.y[i]; r.z = in.z[i];
+    r.vx = in.vx[i]; r.vy = in.vy[i]; r.vz = in.vz[i];
+    r.t = in.t[i];
+    r.w = need_w ? in.wavelength[i] : 0.0;
+    const int nstep = plan.nstep;
+    for (int k = 0; k < nstep; k++) {
+        if (r.fail && r.vig) break;   // nothing below can change this ray
+        const bt_step* __restrict__ st = plan.steps + k;
+        const int action = st->action;
+        const int mi = st->medium;
+        apply_interaction(action, st->dr, st->drot, plan.surfaces + st->surface,
+                          mi >= 0 ? plan.media + mi : nullptr, r, niter);
+        const int olen = st->obsc_len;
+        if (olen > 0 && !r.vig)
+            r.vig = obsc_contains(plan.obscs + st->obsc, olen, r.x, r.y);
+    }
+    out.x[i] = r.x; out.y[i] = r.y; out.z[i] = r.z;
+    out.vx[i] = r.vx; out.vy[i] = r.vy; out.vz[i] = r.vz;
+    out.t[i] = r.t;
+    out.vignetted[i] = r.vig;
+    out.failed[i] = r.fail;
+    if (out.wavelength != in.wavelength) out.wavelength[i] = in.wavelength[i];
+    if (out.flux != in.flux) out.flux[i] = in.flux[i];
+}
+
+__global__ __launch_bounds__(kBlock) void k_sag(
+    const bt_surface* __restrict__ s, const double* __restrict__ x,
+    const double* __restrict__ y, double* __restrict__ out, int64_t n)
+{
+    const int64_t i = ray_index();
+    if (i >= n) return;
+    out[i] = surf_sag(s, x[i], y[i]);
+}
+
+__global__ __launch_bounds__(kBlock) void k_normal(
+    const bt_surface* __restrict__ s, const double* __restrict__ x,
+    const double* __restrict__ y, double* __restrict__ out, int64_t n)
+{                                                    // pysrc/surface.cpp:12-21
+    const int64_t i = ray_index();
+    if (i >= n) return;
+    double nx, ny, nz;
+    surf_normal(s, x[i], y[i], nx, ny, nz);
+    out[i] = nx; out[i+n] = ny; out[i+2*n] = nz;
+}
+
+__global__ __launch_bounds__(kBlock) void k_get_n(
+    const bt_medium* __restrict__ m, const double* __restrict__ w,
+    double* __restrict__ out, int64_t n)
+{
+    const int64_t i = ray_index();
+    if (i >= n) return;
+    out[i] = medium_n(m, w[i]);
+}
+
+__global__ __launch_bounds__(kBlock) void k_contains(
+    const bt_obsc* __restrict__ o, int len, const double* __restrict__ x,
+    const double* __restrict__ y, uint8_t* __restrict__ out, int64_t n)
+{
+    const int64_t i = ray_index();
+    if (i >= n) return;
+    out[i] = obsc_contains(o, len, x[i], y[i]) ? 1 : 0;
+}
+
+// ---------------------------------------------------------------------------
+// Host helpers
+// ---------------------------------------------------------------------------
+bool valid_surface_kind(int k) { return k >= BT_PLANE && k <= BT_SUM; }
+
+int step_launch(int action, const bt_surface* surface, const double* dr,
+                const double* drot, const bt_medium* m2, bt_rays rays,
+                int niter, const bt_obsc* obsc, int obsc_len, void* stream,
+                const char* name)
+{
+    if (!surface || !dr || !drot) return fail(BT_E_NULL, std::string(name) + ": null argument");
+    if (!rays_ok(rays)) return fail(BT_E_NULL, std::string(name) + ": incomplete ray columns");
+    if (action == BT_ACT_REFRACT && !m2) return fail(BT_E_NULL, "bt_refract: null medium");
+    if (obsc_len < 0 || (obsc_len > 0 && !obsc))
+        return fail(BT_E_SIZE, std::string(name) + ": bad obscuration program");
+    if (niter < 0) return fail(BT_E_SIZE, std::string(name) + ": negative niter");
+    if (rays.n == 0) return 0;
+    hipLaunchKernelGGL(k_step, dim3(grid_for(rays.n)), dim3(kBlock), 0, as_stream(stream),
+                       action, make_xform(dr, drot), surface, m2, obsc, obsc_len, rays, niter);
+    return check_launch(name);
+}
+
+}  // namespace bt
+
+using namespace bt;
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int bt_abi_version(void) { return BT_ABI_VERSION; }
+
+const char* bt_last_error(void) { return g_last_error.c_str(); }
+
+int bt_struct_sizes(int64_t* sizes, int n)
+{
+    const int64_t all[6] = {(int64_t)sizeof(bt_rays), (int64_t)sizeof(bt_surface),
+                            (int64_t)sizeof(bt_medium), (int64_t)sizeof(bt_obsc),
+                            (int64_t)sizeof(bt_step), (int64_t)sizeof(bt_plan)};
+    if (!sizes) return fail(BT_E_NULL, "bt_struct_sizes: null argument");
+    for (int i = 0; i < n && i < 6; i++) sizes[i] = all[i];
+    return 0;
+}
+
+int bt_device_info(int* cu_count, int* arch_major, int* arch_minor)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return fail((int)e, "hipGetDevice failed");
+    hipDeviceProp_t prop;
+    e = hipGetDeviceProperties(&prop, dev);
+    if (e != hipSuccess) return fail((int)e, "hipGetDeviceProperties failed");
+    if (cu_count) *cu_count = prop.multiProcessorCount;
+    if (arch_major) *arch_major = prop.major;
+    if (arch_minor) *arch_minor = prop.minor;
+    return 0;
+}
+
+int bt_apply_forward_transform(const double* dr, const double* drot,
+                               bt_rays rays, void* stream)
+{
+    if (!dr || !drot || !rays_ok(rays)) return fail(BT_E_NULL, "bt_apply_forward_transform: null argument");
+    if (rays.n == 0) return 0;
+    hipLaunchKernelGGL(k_transform_rays<true>, dim3(grid_for(rays.n)), dim3(kBlock), 0,
+                       as_stream(stream), make_xform(dr, drot), rays);
+    return check_launch("bt_apply_forward_transform");
+}
+
+int bt_apply_reverse_transform(const double* dr, const double* drot,
+                               bt_rays rays, void* stream)
+{
+    if (!dr || !drot || !rays_ok(rays)) return fail(BT_E_NULL, "bt_apply_reverse_transform: null argument");
+    if (rays.n == 0) return 0;
+    hipLaunchKernelGGL(k_transform_rays<false>, dim3(grid_for(rays.n)), dim3(kBlock), 0,
+                       as_stream(stream), make_xform(dr, drot), rays);
+    return check_launch("bt_apply_reverse_transform");
+}
+
+int bt_apply_forward_transform_arrays(const double* dr, const double* drot,
+                                      double* x, double* y, double* z,
+                                      int64_t n, void* stream)
+{
+    if (!dr || !drot || (n > 0 && (!x || !y || !z)))
+        return fail(BT_E_NULL, "bt_apply_forward_transform_arrays: null argument");
+    if (n < 0) return fail(BT_E_SIZE, "bt_apply_forward_transform_arrays: negative n");
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(k_transform_arrays<true>, dim3(grid_for(n)), dim3(kBlock), 0,
+                       as_stream(stream), make_xform(dr, drot), x, y, z, n);
+    return check_launch("bt_apply_forward_transform_arrays");
+}
+
+int bt_apply_reverse_transform_arrays(const double* dr, const double* drot,
+                                      double* x, double* y, double* z,
+                                      int64_t n, void* stream)
+{
+    if (!dr || !drot || (n > 0 && (!x || !y || !z)))
+        return fail(BT_E_NULL, "bt_apply_reverse_transform_arrays: null argument");
+    if (n < 0) return fail(BT_E_SIZE, "bt_apply_reverse_transform_arrays: negative n");
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(k_transform_arrays<false>, dim3(grid_for(n)), dim3(kBlock), 0,
+                       as_stream(stream), make_xform(dr, drot), x, y, z, n);
+    return check_launch("bt_apply_reverse_transform_arrays");
+}
+
+int bt_finish_parallel(const double* dr, const double* drot, const double* vv,
+                       double* x, double* y, double* z, int64_t n, void* stream)
+{
+    if (!dr || !drot || !vv || (n > 0 && (!x || !y || !z)))
+        return fail(BT_E_NULL, "bt_finish_parallel: null argument");
+    if (n < 0) return fail(BT_E_SIZE, "bt_finish_parallel: negative n");
+    if (n == 0) return 0;
+    FinishArgs a;
+    a.xf = make_xform(dr, drot);
+    // batoid.cpp:44-46, evaluated once on the host as the reference does
+    a.vl[0] = -vv[0]*drot[0] - vv[1]*drot[3] - vv[2]*drot[6];
+    a.vl[1] = -vv[0]*drot[1] - vv[1]*drot[4] - vv[2]*drot[7];
+    a.vl[2] = -vv[0]*drot[2] - vv[1]*drot[5] - vv[2]*drot[8];
+    hipLaunchKernelGGL(k_finish_parallel, dim3(grid_for(n)), dim3(kBlock), 0,
+                       as_stream(stream), a, x, y, z, n);
+    return check_launch("bt_finish_parallel");
+}
+
+int bt_intersect(const bt_surface* surface, const double* dr,
+                 const double* drot, bt_rays rays, int niter,
+                 const bt_obsc* obsc, int obsc_len, void* stream)
+{
+    return step_launch(BT_ACT_INTERSECT, surface, dr, drot, nullptr, rays, niter,
+                       obsc, obsc_len, stream, "bt_intersect");
+}
+
+int bt_reflect(const bt_surface* surface, const double* dr,
+               const double* drot, bt_rays rays, int niter,
+               const bt_obsc* obsc, int obsc_len, void* stream)
+{
+    return step_launch(BT_ACT_REFLECT, surface, dr, drot, nullptr, rays, niter,
+                       obsc, obsc_len, stream, "bt_reflect");
+}
+
+int bt_refract(const bt_surface* surface, const double* dr,
+               const double* drot, const bt_medium* m2, bt_rays rays,
+               int niter, const bt_obsc* obsc, int obsc_len, void* stream)
+{
+    return step_launch(BT_ACT_REFRACT, surface, dr, drot, m2, rays, niter,
+                       obsc, obsc_len, stream, "bt_refract");
+}
+
+int bt_obscure(const bt_obsc* obsc, int obsc_len, bt_rays rays, void* stream)
+{
+    if (!obsc || obsc_len <= 0) return fail(BT_E_NULL, "bt_obscure: empty obscuration program");
+    if (!rays_ok(rays)) return fail(BT_E_NULL, "bt_obscure: incomplete ray columns");
+    if (rays.n == 0) return 0;
+    hipLaunchKernelGGL(k_obscure, dim3(grid_for(rays.n)), dim3(kBlock), 0,
+                       as_stream(stream), obsc, obsc_len, rays);
+    return check_launch("bt_obscure");
+}
+
+int bt_trace_plan(bt_plan plan, bt_rays in, bt_rays out, int niter, void* stream)
+{
+    if (plan.nstep < 0) return fail(BT_E_PLAN, "bt_trace_plan: negative step count");
+    if (plan.nstep > 0 && (!plan.steps || !plan.surfaces))
+        return fail(BT_E_PLAN, "bt_trace_plan: plan tables missing");
+    if (!rays_ok(in) || !rays_ok(out)) return fail(BT_E_NULL, "bt_trace_plan: incomplete ray columns");
+    if (in.n != out.n) return fail(BT_E_SIZE, "bt_trace_plan: in/out sizes differ");
+    if (niter < 0) return fail(BT_E_SIZE, "bt_trace_plan: negative niter");
+    if (in.n == 0) return 0;
+    const int need_w = (plan.flags & BT_PLAN_NO_REFRACT) ? 0 : 1;
+    hipLaunchKernelGGL(k_plan, dim3(grid_for(in.n)), dim3(kBlock), 0, as_stream(stream),
+                       plan, in, out, niter, need_w);
+    return check_launch("bt_trace_plan");
+}
+
+int bt_surface_sag(const bt_surface* surface, const double* x, const double* y,
+                   double* out, int64_t n, void* stream)
+{
+    if (!surface || (n > 0 && (!x || !y || !out))) return fail(BT_E_NULL, "bt_surface_sag: null argument");
+    if (n < 0) return fail(BT_E_SIZE, "bt_surface_sag: negative n");
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(k_sag, dim3(grid_for(n)), dim3(kBlock), 0, as_stream(stream),
+                       surface, x, y, out, n);
+    return check_launch("bt_surface_sag");
+}
+
+int bt_surface_normal(const bt_surface* surface, const double* x,
+                      const double* y, double* normal_out, int64_t n,
+                      void* stream)
+{
+    if (!surface || (n > 0 && (!x || !y || !normal_out)))
+        return fail(BT_E_NULL, "bt_surface_normal: null argument");
+    if (n < 0) return fail(BT_E_SIZE, "bt_surface_normal: negative n");
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(k_normal, dim3(grid_for(n)), dim3(kBlock), 0, as_stream(stream),
+                       surface, x, y, normal_out, n);
+    return check_launch("bt_surface_normal");
+}
+
+int bt_medium_get_n(const bt_medium* medium, const double* wavelength,
+                    double* out, int64_t n, void* stream)
+{
+    if (!medium || (n > 0 && (!wavelength || !out))) return fail(BT_E_NULL, "bt_medium_get_n: null argument");
+    if (n < 0) return fail(BT_E_SIZE, "bt_medium_get_n: negative n");
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(k_get_n, dim3(grid_for(n)), dim3(kBlock), 0, as_stream(stream),
+                       medium, wavelength, out, n);
+    return check_launch("bt_medium_get_n");
+}
+
+int bt_obsc_contains(const bt_obsc* obsc, int obsc_len, const double* x,
+                     const double* y, uint8_t* out, int64_t n, void* stream)
+{
+    if (!obsc || obsc_len <= 0 || (n > 0 && (!x || !y || !out)))
+        return fail(BT_E_NULL, "bt_obsc_contains: null argument");
+    if (n < 0) return fail(BT_E_SIZE, "bt_obsc_contains: negative n");
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(k_contains, dim3(grid_for(n)), dim3(kBlock), 0, as_stream(stream),
+                       obsc, obsc_len, x, y, out, n);
+    return check_launch("bt_obsc_contains");
+}
+
+}  // extern "C"

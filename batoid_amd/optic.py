@@ -1,0 +1,372 @@
+"""Optical elements and systems (batoid/optic.py).
+
+``Interface`` subclasses choose the interaction (intersect / reflect /
+refract); ``CompoundOptic.trace`` compiles its unskipped Interfaces into a
+plan and traces every ray through the whole optic in one fused GPU kernel
+(per-ray results identical to calling each Interface in turn)."""
+from functools import cached_property
+
+from . import _lib, _device
+from .constants import globalCoordSys, vacuum
+from .obscuration import ObscNegation, ObscCircle, ObscAnnulus
+
+
+_DERIVED = ('path', 'itemDict', '_names', 'R_inner', 'R_outer', 'inRadius',
+            'outRadius', '_plan_cache')
+
+
+def _init_kwargs(obj, drop=()):
+    """Constructor keywords that rebuild ``obj`` (minus derived state)."""
+    return {k: v for k, v in obj.__dict__.items()
+            if k not in _DERIVED and k not in drop}
+
+
+class Optic:
+    """Base class: name, frame, media, skip flag, plus any extra attributes
+    (backDist, pupilSize, stopSurface, ...) as keyword arguments
+    (optic.py:13-47)."""
+
+    def __init__(self, name=None, coordSys=globalCoordSys, inMedium=vacuum,
+                 outMedium=vacuum, skip=False, **kwargs):
+        self.name = name
+        self.coordSys = coordSys
+        self.inMedium = inMedium
+        self.outMedium = outMedium
+        self.skip = False
+        kwargs.pop('itemDict', None)
+        self.__dict__.update(**kwargs)
+
+    @classmethod
+    def fromYaml(cls, filename):
+        """Load an optic from a batoid YAML description (optic.py:59-95)."""
+        from .parse import load_yaml, parse_optic
+        return parse_optic(load_yaml(filename)['opticalSystem'])
+
+    def _repr_helper(self):
+        out = ""
+        if self.name is not None:
+            out += ", name={!r}".format(self.name)
+        out += ", coordSys={!r}".format(self.coordSys)
+        if self.inMedium != vacuum:
+            out += ", inMedium={!r}".format(self.inMedium)
+        if self.outMedium != vacuum:
+            out += ", outMedium={!r}".format(self.outMedium)
+        return out
+
+
+class Interface(Optic):
+    """A single surface with an optional obscuration; rays pass through
+    (optic.py:98-140, 541-543)."""
+
+    def __init__(self, surface, obscuration=None, **kwargs):
+        Optic.__init__(self, **kwargs)
+        self.path = [self.name]
+        self.surface = surface
+        self.obscuration = obscuration
+        self.inRadius = self.R_inner
+        self.outRadius = self.R_outer
+
+    def trace(self, rv, reverse=False):
+        """Interact and then obscure, in place (optic.py:391-424); both run in
+        one GPU pass."""
+        self._interact(rv, reverse, self.obscuration)
+        return rv
+
+    def _interact(self, rv, reverse, obscuration):
+        from .trace import intersect
+        return intersect(self.surface, rv, self.coordSys, obscuration=obscuration)
+
+    def interact(self, rv, reverse=False):
+        return self._interact(rv, reverse, None)
+
+    def intersect(self, rv, reverse=False):
+        from .trace import intersect
+        return intersect(self.surface, rv, self.coordSys)
+
+    def traceFull(self, rv, reverse=False):
+        """{name: {'name', 'in', 'out'}} for this single Interface."""
+        from collections import OrderedDict
+        out = OrderedDict()
+        rin = rv.copy()
+        self.trace(rv, reverse=reverse)
+        out[self.name] = {'name': self.name, 'in': rin, 'out': rv.copy()}
+        return out
+
+    def clearObscuration(self, unless=()):
+        if self.name not in unless:
+            self.obscuration = None
+
+    @cached_property
+    def R_outer(self):
+        obsc = self.obscuration
+        if obsc is not None:
+            if isinstance(obsc, ObscNegation):
+                obsc = obsc.original
+            if isinstance(obsc, ObscAnnulus):
+                return obsc.outer
+            if isinstance(obsc, ObscCircle):
+                return obsc.radius
+        return None
+
+    @cached_property
+    def R_inner(self):
+        obsc = self.obscuration
+        if obsc is not None:
+            if isinstance(obsc, ObscNegation):
+                obsc = obsc.original
+            if isinstance(obsc, ObscAnnulus):
+                return obsc.inner
+            if isinstance(obsc, ObscCircle):
+                return 0.0
+        return None
+
+    def withSurface(self, surface):
+        """Copy of this Interface with ``surface`` replaced (optic.py:698-715)."""
+        ret = self.__class__.__new__(self.__class__)
+        kw = _init_kwargs(self, ('surface',))
+        ret.__init__(surface, **kw)
+        return ret
+
+    def withPerturbedSurface(self, perturbation):
+        """Copy with ``surface + perturbation`` (optic.py:717-731)."""
+        return self.withSurface(self.surface + perturbation)
+
+    def __hash__(self):
+        return hash((self.__class__.__name__, self.surface, self.obscuration,
+                     self.name, self.inMedium, self.outMedium, self.coordSys))
+
+    def __eq__(self, other):
+        if not self.__class__ == other.__class__:
+            return False
+        return (self.surface == other.surface and self.obscuration == other.obscuration
+                and self.name == other.name and self.inMedium == other.inMedium
+                and self.outMedium == other.outMedium and self.coordSys == other.coordSys)
+
+    def __ne__(self, other):
+        return not (self == other)
+
+    def __repr__(self):
+        out = "{!s}({!r}".format(self.__class__.__name__, self.surface)
+        if self.obscuration is not None:
+            out += ", obscuration={!r}".format(self.obscuration)
+        return out + Optic._repr_helper(self) + ")"
+
+
+class RefractiveInterface(Interface):
+    """Refracts from inMedium into outMedium (reverse swaps them)
+    (optic.py:762-797)."""
+
+    def _interact(self, rv, reverse, obscuration):
+        from .trace import refract
+        m1, m2 = (self.outMedium, self.inMedium) if reverse else (self.inMedium, self.outMedium)
+        return refract(self.surface, rv, m1, m2, self.coordSys, obscuration=obscuration)
+
+    def refract(self, rv, reverse=False):
+        return self.interact(rv, reverse=reverse)
+
+
+class Mirror(Interface):
+    """Reflects (optic.py:800-823)."""
+
+    def _interact(self, rv, reverse, obscuration):
+        from .trace import reflect
+        return reflect(self.surface, rv, self.coordSys, obscuration=obscuration)
+
+    def reflect(self, rv, reverse=False):
+        return self.interact(rv, reverse=reverse)
+
+
+class Detector(Interface):
+    """Rays stop here (intersect only) (optic.py:826-845)."""
+
+
+class Baffle(Interface):
+    """Rays pass straight through; only vignetting applies (optic.py:848-864)."""
+
+
+class OPDScreen(Interface):
+    """Surface plus an optical-path-difference screen (optic.py:867-930).
+    Tracing through it (refractScreen) is outside the GPU trace path."""
+
+    def __init__(self, surface, screen, **kwargs):
+        Interface.__init__(self, surface, **kwargs)
+        self.screen = screen
+
+    def _interact(self, rv, reverse, obscuration):
+        raise NotImplementedError("OPDScreen is outside the GPU trace path")
+
+
+class CompoundOptic(Optic):
+    """A sequence of optics traced in order (optic.py:933-1083)."""
+
+    def __init__(self, items, **kwargs):
+        Optic.__init__(self, **kwargs)
+        self.items = tuple(items)
+        self.path = [name for item in self.items for name in item.path]
+
+    @cached_property
+    def itemDict(self):
+        out = {self.name: self}
+        for item in self.items:
+            out[self.name + '.' + item.name] = item
+            if hasattr(item, 'itemDict'):
+                for k, v in item.itemDict.items():
+                    out[self.name + '.' + k] = v
+        return out
+
+    @cached_property
+    def _names(self):
+        nameDict = {}
+        duplicates = set()
+        for k in self.itemDict.keys():
+            tokens = k.split('.')
+            shortNames = [tokens[-1]]
+            for token in reversed(tokens[:-1]):
+                shortNames.append('.'.join([token, shortNames[-1]]))
+            for shortName in shortNames:
+                if shortName in nameDict:
+                    duplicates.add(shortName)
+                else:
+                    nameDict[shortName] = k
+        for name in duplicates:
+            del nameDict[name]
+        return nameDict
+
+    def __getitem__(self, key):
+        try:
+            return self.itemDict[key]
+        except KeyError:
+            try:
+                return self.itemDict[self._names[key]]
+            except KeyError:
+                raise ValueError("Cannot find item {}".format(key))
+
+    def _skips(self):
+        flags = []
+
+        def visit(o):
+            flags.append(o.skip)
+            for it in getattr(o, 'items', ()):
+                visit(it)
+        visit(self)
+        return tuple(flags)
+
+    def _plan(self, coordSys_in, reverse):
+        from .plan import PlanSpec, DevicePlan, collect_steps
+        key = (coordSys_in, reverse, self._skips())
+        cache = self.__dict__.setdefault('_plan_cache', {})
+        plan = cache.get(key)
+        if plan is None:
+            plan = DevicePlan(PlanSpec(collect_steps(self, reverse), coordSys_in))
+            cache[key] = plan
+        return plan
+
+    def trace(self, rv, reverse=False, path=None):
+        """Trace rays through every unskipped item, in place
+        (optic.py:1013-1083).  Without ``path`` the whole optic runs as one
+        fused GPU kernel; with ``path`` the items are traced one by one in
+        the requested order."""
+        from .global_vars import _batoid_niter
+        if path is not None:
+            return self._trace_path(rv, path)
+        plan = self._plan(rv.coordSys, reverse)
+        if plan.struct.nstep == 0:
+            return rv
+        rays = rv._rays
+        _lib.call("bt_trace_plan", plan.struct, rays, rays, _batoid_niter, _device.stream())
+        rv.coordSys = plan.spec.coordSys_out
+        return rv
+
+    def _trace_path(self, rv, path):
+        i = 0
+        nominalOrder = {}
+        for name in path:
+            if name not in nominalOrder:
+                nominalOrder[name] = i
+                i += 1
+        direction = "forward"
+        for i in range(len(path)):
+            currentName = path[i]
+            item = self[currentName]
+            if i == len(path) - 1:
+                nextDirection = "reverse" if nominalOrder[path[i]] == 0 else direction
+            else:
+                nextName = path[i + 1]
+                nextDirection = ("reverse" if nominalOrder[nextName] < nominalOrder[currentName]
+                                 else "forward")
+            if direction == nextDirection:
+                item.trace(rv, reverse=(direction == "reverse"))
+            else:
+                direction = nextDirection
+                from .trace import reflect
+                reflect(item.surface, rv, item.coordSys, obscuration=item.obscuration)
+        return rv
+
+    def traceFull(self, rv, reverse=False, path=None):
+        """Per-Interface history {name: {'name', 'in', 'out'}}
+        (optic.py:1085-1182)."""
+        from collections import OrderedDict
+        from .plan import collect_steps
+        if path is not None:
+            raise NotImplementedError("traceFull with path is not supported")
+        out = OrderedDict()
+        for iface, _, _ in collect_steps(self, reverse):
+            rin = rv.copy()
+            iface.trace(rv, reverse=reverse)
+            out[iface.name] = {'name': iface.name, 'in': rin, 'out': rv.copy()}
+        return out
+
+    def clearObscuration(self, unless=()):
+        for item in self.items:
+            item.clearObscuration(unless=unless)
+
+    def withSurface(self, name, surface):
+        """Copy with the surface of item ``name`` replaced (optic.py:1695-1735)."""
+        if name == self.name:
+            raise ValueError("Cannot replace surface of CompoundOptic")
+        item = self[name]
+        return self._replaced(item, item.withSurface(surface))
+
+    def _replaced(self, old, new):
+        items = []
+        for it in self.items:
+            if it is old:
+                items.append(new)
+            elif isinstance(it, CompoundOptic):
+                items.append(it._replaced(old, new))
+            else:
+                items.append(it)
+        ret = self.__class__.__new__(self.__class__)
+        ret.__init__(items, **_init_kwargs(self, ('items',)))
+        return ret
+
+    def withPerturbedSurface(self, name, perturbation):
+        return self.withSurface(name, self[name].surface + perturbation)
+
+    def __eq__(self, other):
+        if not self.__class__ == other.__class__:
+            return False
+        return (self.items == other.items and self.name == other.name
+                and self.inMedium == other.inMedium and self.outMedium == other.outMedium
+                and self.coordSys == other.coordSys)
+
+    def __ne__(self, other):
+        return not (self == other)
+
+    def __hash__(self):
+        return hash((self.__class__.__name__, self.items, self.name, self.inMedium,
+                     self.outMedium, self.coordSys))
+
+    def __repr__(self):
+        out = "{!s}([".format(self.__class__.__name__)
+        out += ",".join([repr(item) for item in self.items])
+        return out + "]" + Optic._repr_helper(self) + ")"
+
+
+class Lens(CompoundOptic):
+    """Two Interfaces around a medium (optic.py:1821-1834)."""
+
+    def __init__(self, items, **kwargs):
+        CompoundOptic.__init__(self, items, **kwargs)
+        assert len(items) == 2
+        assert items[0].outMedium == items[1].inMedium

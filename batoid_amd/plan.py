@@ -1,0 +1,130 @@
+"""Optic -> trace plan compiler.
+
+``CompoundOptic.trace`` (optic.py:1013-1049) visits its Interfaces in order
+and each one transforms the rays into its frame, intersects/reflects/
+refracts and then obscures (optic.py:391-424).  A plan is that sequence
+flattened into tables the fused GPU kernel walks per ray:
+
+* steps: action, surface index, outgoing-medium index, obscuration
+  program, and the CoordTransform (dr, drot) from the previous step's frame;
+* surfaces / media / obscuration records, shared between steps that use
+  the same object (so a Bicubic table is uploaded once).
+"""
+import numpy as np
+
+from . import _lib
+from .coordTransform import CoordTransform
+
+
+def _key(obj):
+    """Table key: equal objects share one uploaded record set."""
+    try:
+        hash(obj)
+        return (type(obj), obj)
+    except TypeError:
+        return (type(obj), id(obj))
+
+
+class PlanSpec:
+    """Host-side plan: ctypes records plus the numpy tables they point to."""
+
+    def __init__(self, steps, coordSys_in):
+        self.surf_recs, self.surf_arrs = [], []
+        self.med_recs, self.med_arrs = [], []
+        self.ob_recs, self.ob_arrs = [], []
+        self.steps = []
+        self.interfaces = []
+        surf_index, med_index, ob_index = {}, {}, {}
+        refracts = False
+        cs = coordSys_in
+        for iface, action, m2 in steps:
+            key = _key(iface.surface)
+            if key not in surf_index:
+                surf_index[key] = len(self.surf_recs)
+                r, a = iface.surface._records()
+                self.surf_recs += r
+                self.surf_arrs += a
+            mi = -1
+            if action == _lib.ACT_REFRACT:
+                refracts = True
+                mkey = _key(m2)
+                if mkey not in med_index:
+                    med_index[mkey] = len(self.med_recs)
+                    r, a = m2._records()
+                    self.med_recs += r
+                    self.med_arrs += a
+                mi = med_index[mkey]
+            oi, ol = -1, 0
+            if iface.obscuration is not None:
+                okey = _key(iface.obscuration)
+                if okey not in ob_index:
+                    r, a = iface.obscuration._records()
+                    ob_index[okey] = (len(self.ob_recs), len(r))
+                    self.ob_recs += r
+                    self.ob_arrs += a
+                oi, ol = ob_index[okey]
+            ct = CoordTransform(cs, iface.coordSys)
+            st = _lib.BtStep()
+            st.action = action
+            st.surface = surf_index[key]
+            st.medium = mi
+            st.obsc = oi
+            st.obsc_len = ol
+            for i, v in enumerate(ct.dr):
+                st.dr[i] = float(v)
+            for i, v in enumerate(np.asarray(ct.drot).ravel()):
+                st.drot[i] = float(v)
+            self.steps.append(st)
+            self.interfaces.append(iface)
+            cs = iface.coordSys
+        self.coordSys_in = coordSys_in
+        self.coordSys_out = cs
+        self.flags = 0 if refracts else _lib.PLAN_NO_REFRACT
+
+
+class DevicePlan:
+    """A PlanSpec uploaded to GPU memory; ``struct`` is the bt_plan."""
+
+    def __init__(self, spec):
+        from ._device import Materialized, upload_bytes
+        self.spec = spec
+        self.surfaces = Materialized(spec.surf_recs, spec.surf_arrs) if spec.surf_recs else None
+        self.media = Materialized(spec.med_recs, spec.med_arrs) if spec.med_recs else None
+        self.obscs = Materialized(spec.ob_recs, spec.ob_arrs) if spec.ob_recs else None
+        self.steps = upload_bytes(_lib.struct_array_bytes(spec.steps))
+        p = _lib.BtPlan()
+        p.nstep = len(spec.steps)
+        p.flags = spec.flags
+        p.steps = self.steps.data_ptr() if spec.steps else None
+        p.surfaces = self.surfaces.buffer.data_ptr() if self.surfaces else None
+        p.media = self.media.buffer.data_ptr() if self.media else None
+        p.obscs = self.obscs.buffer.data_ptr() if self.obscs else None
+        self.struct = p
+
+
+def collect_steps(optic, reverse=False):
+    """(interface, action, outgoing medium) for every unskipped Interface,
+    in trace order (optic.py:1045-1049, 541-543, 776-781, 814-816)."""
+    from .optic import CompoundOptic, Interface, Mirror, RefractiveInterface, OPDScreen
+    out = []
+
+    def visit(item):
+        if isinstance(item, CompoundOptic):
+            items = item.items if not reverse else tuple(reversed(item.items))
+            for sub in items:
+                if not sub.skip:
+                    visit(sub)
+        elif isinstance(item, OPDScreen):
+            raise NotImplementedError("OPDScreen is outside the GPU trace path")
+        elif isinstance(item, RefractiveInterface):
+            m2 = item.inMedium if reverse else item.outMedium
+            out.append((item, _lib.ACT_REFRACT, m2))
+        elif isinstance(item, Mirror):
+            out.append((item, _lib.ACT_REFLECT, None))
+        elif isinstance(item, Interface):
+            out.append((item, _lib.ACT_INTERSECT, None))
+        else:
+            raise TypeError(f"cannot trace {type(item).__name__}")
+
+    visit(optic)
+    return out

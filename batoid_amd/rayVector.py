@@ -1,0 +1,455 @@
+"""RayVector: structure-of-arrays ray columns resident in GPU memory.
+
+Same constructor, generators and accessors as batoid/rayVector.py.  The
+11 columns (x, y, z, vx, vy, vz, t, wavelength, flux as FP64; vignetted,
+failed as u8) are torch device tensors; the HIP library receives their raw
+pointers.  Column accessors (``rv.x`` ...) return host numpy copies, the
+counterpart of the reference's syncToHost on property access
+(rayVector.py:1083-1176); ``rv.torch(name)`` gives the device column
+itself without a copy.
+"""
+from numbers import Real
+
+import numpy as np
+import torch
+
+from . import _lib, _device
+from .constants import globalCoordSys, vacuum
+from .coordTransform import CoordTransform
+from .utils import fieldToDirCos, hexapolar
+
+_FCOLS = ("x", "y", "z", "vx", "vy", "vz", "t", "wavelength", "flux")
+_BCOLS = ("vignetted", "failed")
+_COLS = _FCOLS + _BCOLS
+
+
+class RayVector:
+    """Rays as device columns.  Arguments broadcast against each other and
+    are copied (rayVector.py:21-77)."""
+
+    def __init__(self, x, y, z, vx, vy, vz, t=0.0, wavelength=0.0, flux=1.0,
+                 vignetted=False, failed=False, coordSys=globalCoordSys):
+        shape = np.broadcast(x, y, z, vx, vy, vz, t, wavelength, flux,
+                             vignetted, failed).shape
+        if shape == ():
+            shape = (1,)
+        cols = {}
+        for name, val in zip(_FCOLS, (x, y, z, vx, vy, vz, t, wavelength, flux)):
+            if isinstance(val, torch.Tensor):
+                val = val.detach().cpu().numpy()
+            cols[name] = _device.f64(np.broadcast_to(np.asarray(val, float), shape).ravel())
+        for name, val in zip(_BCOLS, (vignetted, failed)):
+            if isinstance(val, torch.Tensor):
+                val = val.detach().cpu().numpy()
+            cols[name] = _device.u8(np.broadcast_to(np.asarray(val, bool), shape).ravel())
+        self._cols = cols
+        self.coordSys = coordSys
+
+    @staticmethod
+    def _directInit(cols, coordSys):
+        """Wrap existing device columns (no copy)."""
+        ret = RayVector.__new__(RayVector)
+        ret._cols = cols
+        ret.coordSys = coordSys
+        return ret
+
+    @classmethod
+    def fromTorch(cls, x, y, z, vx, vy, vz, t, wavelength, flux, vignetted, failed,
+                  coordSys=globalCoordSys):
+        """Adopt device tensors as columns without copying (they must be
+        contiguous float64 / uint8 tensors of equal length on the GPU)."""
+        cols = dict(zip(_COLS, (x, y, z, vx, vy, vz, t, wavelength, flux,
+                                vignetted, failed)))
+        n = cols["x"].numel()
+        for k, v in cols.items():
+            want = torch.uint8 if k in _BCOLS else torch.float64
+            if v.dtype != want or not v.is_cuda or not v.is_contiguous() or v.numel() != n:
+                raise ValueError(f"column {k} must be a contiguous {want} CUDA tensor of length {n}")
+        return cls._directInit(cols, coordSys)
+
+    # ---- C-ABI view -------------------------------------------------------
+    @property
+    def _rays(self):
+        r = _lib.BtRays()
+        for k in _COLS:
+            setattr(r, k, self._cols[k].data_ptr())
+        r.n = len(self)
+        return r
+
+    def torch(self, name):
+        """The device column ``name`` itself (no copy)."""
+        return self._cols[name]
+
+    def __len__(self):
+        return self._cols["t"].numel()
+
+    # ---- host accessors ---------------------------------------------------
+    def _host(self, name):
+        a = _device.to_numpy(self._cols[name])
+        return a.astype(bool) if name in _BCOLS else a
+
+    x = property(lambda self: self._host("x"), doc="x positions [m]")
+    y = property(lambda self: self._host("y"), doc="y positions [m]")
+    z = property(lambda self: self._host("z"), doc="z positions [m]")
+    vx = property(lambda self: self._host("vx"), doc="x velocity [c]")
+    vy = property(lambda self: self._host("vy"), doc="y velocity [c]")
+    vz = property(lambda self: self._host("vz"), doc="z velocity [c]")
+    t = property(lambda self: self._host("t"), doc="optical path length [m]")
+    wavelength = property(lambda self: self._host("wavelength"), doc="vacuum wavelength [m]")
+    flux = property(lambda self: self._host("flux"), doc="flux")
+    vignetted = property(lambda self: self._host("vignetted"), doc="vignetted flags")
+    failed = property(lambda self: self._host("failed"), doc="failed flags")
+
+    @property
+    def r(self):
+        return torch.stack([self._cols[k] for k in ("x", "y", "z")], dim=1).cpu().numpy()
+
+    @property
+    def v(self):
+        return torch.stack([self._cols[k] for k in ("vx", "vy", "vz")], dim=1).cpu().numpy()
+
+    @property
+    def k(self):
+        """Wavevectors 2 pi v / (lambda |v|^2) in rad/m (rayVector.py:1178-1190)."""
+        v = self.v
+        out = 2 * np.pi * v
+        out /= self.wavelength[:, None]
+        out /= np.sum(v * v, axis=-1)[:, None]
+        return out
+
+    kx = property(lambda self: self.k[:, 0])
+    ky = property(lambda self: self.k[:, 1])
+    kz = property(lambda self: self.k[:, 2])
+
+    @property
+    def omega(self):
+        return 2 * np.pi / self.wavelength
+
+    # ---- device operations --------------------------------------------------
+    def positionAtTime(self, t):
+        """Positions at time ``t``, shape (n, 3) (rayVector.cpp:26-54)."""
+        n = len(self)
+        out = _device.empty_f64(3 * n)
+        _lib.call("bt_position_at_time", self._rays, float(t), _device.ptr(out[:n]),
+                  _device.ptr(out[n:2 * n]), _device.ptr(out[2 * n:]), _device.stream())
+        return _device.to_numpy(out).reshape(3, n).T
+
+    def propagate(self, t):
+        """Propagate every ray to time ``t`` in place (rayVector.cpp:56-82)."""
+        _lib.call("bt_propagate_in_place", self._rays, float(t), _device.stream())
+        return self
+
+    def phase(self, r, t):
+        """Plane-wave phases at position ``r`` and time ``t``."""
+        out = _device.empty_f64(len(self))
+        _lib.call("bt_phase", self._rays, float(r[0]), float(r[1]), float(r[2]),
+                  float(t), _device.ptr(out), _device.stream())
+        return _device.to_numpy(out)
+
+    def amplitude(self, r, t):
+        """Complex amplitudes exp(i phase) at ``r``, ``t``."""
+        out = _device.empty_f64(2 * len(self))
+        _lib.call("bt_amplitude", self._rays, float(r[0]), float(r[1]), float(r[2]),
+                  float(t), _device.ptr(out), _device.stream())
+        return _device.to_numpy(out).view(np.complex128)
+
+    def sumAmplitude(self, r, t, ignoreVignetted=True):
+        """Flux-weighted sum of amplitudes over unfailed (and, by default,
+        unvignetted) rays (rayVector.cpp:169-222)."""
+        lib = _lib.load()
+        ws = torch.empty(int(lib.bt_sum_amplitude_workspace_bytes(len(self))),
+                         dtype=torch.uint8, device=_device.device())
+        out = _device.empty_f64(2)
+        _lib.call("bt_sum_amplitude", self._rays, float(r[0]), float(r[1]), float(r[2]),
+                  float(t), int(bool(ignoreVignetted)), _device.ptr(out), _device.ptr(ws),
+                  _device.stream())
+        o = _device.to_numpy(out)
+        return complex(o[0], o[1])
+
+    def toCoordSys(self, coordSys):
+        """Express the rays in ``coordSys`` (in place)."""
+        from .trace import applyForwardTransform
+        applyForwardTransform(CoordTransform(self.coordSys, coordSys), self)
+        return self
+
+    def copy(self):
+        return RayVector._directInit({k: v.clone() for k, v in self._cols.items()},
+                                     self.coordSys.copy())
+
+    def __getitem__(self, idx):
+        n = len(self)
+        if isinstance(idx, int):
+            if idx >= n or idx < -n:
+                raise IndexError(f"index {idx} is out of bounds for axis 0 with size {n}")
+            idx = slice(idx % n, idx % n + 1)
+        if isinstance(idx, slice):
+            cols = {k: v[idx].clone() for k, v in self._cols.items()}
+        else:
+            sel = torch.as_tensor(np.asarray(idx), device=_device.device())
+            if sel.dtype == torch.bool:
+                sel = torch.nonzero(sel).flatten()
+            cols = {k: v[sel].contiguous() for k, v in self._cols.items()}
+        return RayVector._directInit(cols, self.coordSys)
+
+    def __eq__(self, rhs):
+        if not isinstance(rhs, RayVector) or len(self) != len(rhs):
+            return False
+        # NaN != NaN, as the reference's elementwise == (dualView.cpp:49-68)
+        return all(bool(torch.equal(self._cols[k], rhs._cols[k])) if k in _BCOLS
+                   else bool((self._cols[k] == rhs._cols[k]).all())
+                   for k in _COLS)
+
+    def __ne__(self, rhs):
+        return not (self == rhs)
+
+    def __repr__(self):
+        return (f"RayVector({self.x!r}, {self.y!r}, {self.z!r}, {self.vx!r}, {self.vy!r}, "
+                f"{self.vz!r}, {self.t!r}, {self.wavelength!r}, {self.flux!r}, "
+                f"{self.vignetted!r}, {self.failed!r}, {self.coordSys!r})")
+
+    def __getstate__(self):
+        return tuple(self._host(k) for k in _COLS) + (self.coordSys,)
+
+    def __setstate__(self, args):
+        self._cols = {k: (_device.u8(a) if k in _BCOLS else _device.f64(a))
+                      for k, a in zip(_COLS, args[:-1])}
+        self.coordSys = args[-1]
+
+    # ---- generators ----------------------------------------------------------
+    @staticmethod
+    def _optic_defaults(optic, backDist, medium, stopSurface, coordSys):
+        from .optic import Interface
+        from .surface import Plane
+        if optic is not None:
+            if backDist is None:
+                backDist = optic.backDist
+            if medium is None:
+                medium = optic.inMedium
+            if stopSurface is None:
+                stopSurface = getattr(optic, "stopSurface", None)
+            if coordSys is None:
+                coordSys = optic.coordSys
+        if backDist is None:
+            backDist = 40.0
+        if stopSurface is None:
+            stopSurface = Interface(Plane())
+        if medium is None:
+            medium = vacuum
+        if coordSys is None:
+            coordSys = globalCoordSys
+        return backDist, medium, stopSurface, coordSys
+
+    @classmethod
+    def _onStop(cls, x, y, stopSurface, coordSys, wavelength, medium, backDist,
+                source, dirCos, flux):
+        """Place rays on the stop surface at (x, y), move them to ``coordSys``
+        and back them up to the entrance plane (rayVector.py:438-444)."""
+        from .trace import applyForwardTransformArrays
+        x = np.ascontiguousarray(x, dtype=float)
+        y = np.ascontiguousarray(y, dtype=float)
+        z = stopSurface.surface.sag(x, y)
+        xd, yd, zd = _device.f64(x), _device.f64(y), _device.f64(np.atleast_1d(z))
+        transform = CoordTransform(stopSurface.coordSys, coordSys)
+        applyForwardTransformArrays(transform, xd, yd, zd)
+        n = medium.getN(wavelength)
+        return cls._finish(backDist, source, dirCos, n, xd, yd, zd, wavelength, flux, coordSys)
+
+    @classmethod
+    def _finish(cls, backDist, source, dirCos, n, x, y, z, wavelength, flux, coordSys):
+        """Map rays back to their source plane (rayVector.py:816-858)."""
+        if source is not None:
+            raise NotImplementedError("rays from a finite source point are not supported")
+        npts = x.numel()
+        vv = np.array(dirCos, dtype=float, copy=True)
+        vv /= n * np.sqrt(np.dot(vv, vv))
+        zhat = -n * vv
+        xhat = np.cross(np.array([1.0, 0.0, 0.0]), zhat)
+        xhat /= np.sqrt(np.dot(xhat, xhat))
+        yhat = np.cross(xhat, zhat)
+        origin = zhat * backDist
+        rot = np.stack([xhat, yhat, zhat]).T
+        _lib.call("bt_finish_parallel", _lib.dvec(origin), _lib.dvec(rot.ravel()),
+                  _lib.dvec(vv), _device.ptr(x), _device.ptr(y), _device.ptr(z), npts,
+                  _device.stream())
+        dev = _device.device()
+        if isinstance(flux, Real):
+            fluxd = torch.full((npts,), float(flux), dtype=torch.float64, device=dev)
+        else:
+            fluxd = _device.f64(np.broadcast_to(np.asarray(flux, float), (npts,)).ravel())
+        cols = {
+            "x": x, "y": y, "z": z,
+            "vx": torch.full((npts,), vv[0], dtype=torch.float64, device=dev),
+            "vy": torch.full((npts,), vv[1], dtype=torch.float64, device=dev),
+            "vz": torch.full((npts,), vv[2], dtype=torch.float64, device=dev),
+            "t": torch.zeros(npts, dtype=torch.float64, device=dev),
+            "wavelength": torch.full((npts,), float(wavelength), dtype=torch.float64, device=dev),
+            "flux": fluxd,
+            "vignetted": torch.zeros(npts, dtype=torch.uint8, device=dev),
+            "failed": torch.zeros(npts, dtype=torch.uint8, device=dev),
+        }
+        return cls._directInit(cols, coordSys)
+
+    @classmethod
+    def asGrid(cls, optic=None, backDist=None, medium=None, stopSurface=None,
+               coordSys=None, wavelength=None, source=None, dirCos=None,
+               theta_x=None, theta_y=None, projection='postel',
+               nx=None, ny=None, dx=None, dy=None, lx=None, ly=None,
+               flux=1, nrandom=None, rng=None):
+        """Rays on a parallelogram grid (or uniformly random within it) on the
+        stop surface (rayVector.py:219-444)."""
+        if optic is not None and lx is None and (nx is None or dx is None):
+            lx = optic.pupilSize
+        backDist, medium, stopSurface, coordSys = cls._optic_defaults(
+            optic, backDist, medium, stopSurface, coordSys)
+        if dirCos is None and source is None:
+            dirCos = fieldToDirCos(theta_x, theta_y, projection=projection)
+        if wavelength is None:
+            raise ValueError("Missing wavelength keyword")
+        if sum(a is not None for a in [nx, dx, lx]) != 2:
+            raise ValueError("Exactly 2 of nx, dx, lx must be specified")
+        if nx is not None and ny is None:
+            ny = nx
+        if dx is not None and dy is None:
+            dy = dx
+        if lx is not None and ly is None:
+            ly = lx if isinstance(lx, Real) else np.dot(np.array([[0, -1], [1, 0]]), lx)
+        if nx is not None and dx is not None:
+            lx = dx * (nx - 2) if nx % 2 == 0 else dx * (nx - 1)
+            ly = dy * (ny - 2) if ny % 2 == 0 else dy * (ny - 1)
+        elif lx is not None and dx is not None:
+            slop = 0.1
+            nx = int((lx / dx + slop) // 2) * 2 + 2
+            ny = int((ly / dy + slop) // 2) * 2 + 2
+        if isinstance(lx, Real):
+            lx = (lx, 0.0)
+        if isinstance(ly, Real):
+            ly = (0.0, ly)
+        if nrandom is not None:
+            if rng is None:
+                rng = np.random.default_rng()
+            elif isinstance(rng, int):
+                rng = np.random.default_rng(rng)
+            xx = rng.uniform(-0.5, 0.5, size=nrandom)
+            yy = rng.uniform(-0.5, 0.5, size=nrandom)
+        else:
+            x_d = 1. if nx <= 2 else (nx - (2 if nx % 2 == 0 else 1)) / nx
+            y_d = 1. if ny <= 2 else (ny - (2 if ny % 2 == 0 else 1)) / ny
+            xx = np.fft.fftshift(np.fft.fftfreq(nx, x_d))
+            yy = np.fft.fftshift(np.fft.fftfreq(ny, y_d))
+            xx, yy = np.meshgrid(xx, yy)
+            xx = xx.ravel()
+            yy = yy.ravel()
+        stack = np.stack([xx, yy])
+        x = np.dot(lx, stack)
+        y = np.dot(ly, stack)
+        return cls._onStop(x, y, stopSurface, coordSys, wavelength, medium, backDist,
+                           source, dirCos, flux)
+
+    @classmethod
+    def asPolar(cls, optic=None, backDist=None, medium=None, stopSurface=None,
+                coordSys=None, wavelength=None, outer=None, inner=None,
+                source=None, dirCos=None, theta_x=None, theta_y=None,
+                projection='postel', nrad=None, naz=None, kfold=6, flux=1,
+                nrandom=None, rng=None):
+        """Rays on a hexapolar grid (or uniformly random) over an annulus of
+        the stop surface (rayVector.py:459-643)."""
+        if optic is not None:
+            if outer is None:
+                outer = optic.pupilSize / 2
+            if inner is None:
+                inner = (optic.pupilSize * optic.pupilObscuration / 2
+                         if hasattr(optic, 'pupilObscuration') else 0.0)
+        elif inner is None:
+            inner = 0.0
+        backDist, medium, stopSurface, coordSys = cls._optic_defaults(
+            optic, backDist, medium, stopSurface, coordSys)
+        if dirCos is None and source is None:
+            dirCos = fieldToDirCos(theta_x, theta_y, projection=projection)
+        if wavelength is None:
+            raise ValueError("Missing wavelength keyword")
+        if nrandom is None:
+            rr, th = hexapolar(outer=outer, inner=inner, nrad=nrad, naz=naz,
+                               kfold=kfold, rth=True)
+        else:
+            if rng is None:
+                rng = np.random.default_rng()
+            elif isinstance(rng, int):
+                rng = np.random.default_rng(rng)
+            rr = np.sqrt(rng.uniform(inner**2, outer**2, size=nrandom))
+            th = rng.uniform(0, 2 * np.pi, size=nrandom)
+        x = rr * np.cos(th)
+        y = rr * np.sin(th)
+        return cls._onStop(x, y, stopSurface, coordSys, wavelength, medium, backDist,
+                           source, dirCos, flux)
+
+    @classmethod
+    def asSpokes(cls, optic=None, backDist=None, medium=None, stopSurface=None,
+                 coordSys=None, wavelength=None, outer=None, inner=0.0,
+                 source=None, dirCos=None, theta_x=None, theta_y=None,
+                 projection='postel', spokes=None, rings=None, spacing='uniform',
+                 flux=1):
+        """Rays on rings x spokes, optionally at Gauss-quadrature radii with
+        quadrature weights as fluxes (rayVector.py:645-814)."""
+        from numbers import Integral
+        if optic is not None and outer is None:
+            outer = optic.pupilSize / 2
+        backDist, medium, stopSurface, coordSys = cls._optic_defaults(
+            optic, backDist, medium, stopSurface, coordSys)
+        if dirCos is None and source is None:
+            dirCos = fieldToDirCos(theta_x, theta_y, projection=projection)
+        if wavelength is None:
+            raise ValueError("Missing wavelength keyword")
+        if isinstance(rings, Integral):
+            if spacing == 'uniform':
+                rings = np.linspace(inner, outer, rings)
+            elif spacing == 'GQ':
+                if spokes is None:
+                    spokes = 2 * rings + 1
+                Li, w = np.polynomial.legendre.leggauss(rings)
+                eps = inner / outer
+                area = np.pi * (1 - eps**2)
+                rings = np.sqrt(eps**2 + (1 + Li) * (1 - eps**2) / 2) * outer
+                flux = w * area / (2 * spokes)
+        if isinstance(spokes, Integral):
+            spokes = np.linspace(0, 2 * np.pi, spokes, endpoint=False)
+        rings, spokes = np.meshgrid(rings, spokes)
+        flux = np.broadcast_to(flux, rings.shape).ravel()
+        rings = rings.ravel()
+        spokes = spokes.ravel()
+        x = rings * np.cos(spokes)
+        y = rings * np.sin(spokes)
+        return cls._onStop(x, y, stopSurface, coordSys, wavelength, medium, backDist,
+                           source, dirCos, flux)
+
+    @classmethod
+    def fromStop(cls, x, y, optic=None, backDist=None, medium=None,
+                 stopSurface=None, coordSys=None, wavelength=None, source=None,
+                 dirCos=None, theta_x=None, theta_y=None, projection='postel',
+                 flux=1):
+        """Rays that would cross the stop surface at (x, y)
+        (rayVector.py:860-982)."""
+        backDist, medium, stopSurface, coordSys = cls._optic_defaults(
+            optic, backDist, medium, stopSurface, coordSys)
+        if dirCos is None and source is None:
+            dirCos = fieldToDirCos(theta_x, theta_y, projection=projection)
+        if wavelength is None:
+            raise ValueError("Missing wavelength keyword")
+        x = np.atleast_1d(x).astype(float, copy=False)
+        y = np.atleast_1d(y).astype(float, copy=False)
+        return cls._onStop(x, y, stopSurface, coordSys, wavelength, medium, backDist,
+                           source, dirCos, flux)
+
+    @classmethod
+    def asFan(cls, nx=None, ny=None, **kwargs):
+        rvs = []
+        if nx > 1:
+            rvs.append(RayVector.asGrid(nx=nx, ny=1, **kwargs))
+        if ny > 1:
+            rvs.append(RayVector.asGrid(nx=1, ny=ny, **kwargs))
+        return concatenateRayVectors(rvs)
+
+
+def concatenateRayVectors(rvs):
+    """Rays of several RayVectors in one (rayVector.py:1351-1365)."""
+    cols = {k: torch.cat([rv._cols[k] for rv in rvs]) for k in _COLS}
+    return RayVector._directInit(cols, rvs[0].coordSys)

@@ -1,0 +1,304 @@
+/*
+ * batoid_amd.h -- C ABI of the MI355X (gfx950) trace + Huygens-PSF library.
+ *
+ * Every entry point takes plain pointers and sizes.  Ray columns, surface,
+ * medium and obscuration descriptors, plans and outputs are DEVICE pointers
+ * owned by the caller (the Python host keeps them alive); the library never
+ * allocates or frees caller memory.  Calls are asynchronous on the HIP stream
+ * passed as `stream` (NULL = the legacy default stream); the caller
+ * synchronises before reading results.  Each call returns 0 on success, a
+ * negative BT_E_* code for an argument error, or a positive hipError_t for a
+ * launch failure; bt_last_error() returns a message for the most recent
+ * failure on the calling thread.
+ *
+ * Per-ray failure is NOT an error: it is reported through the `failed` and
+ * `vignetted` flag columns with the reference's exact semantics
+ * (src/batoid.cpp:236-268, 478-487 of jmeyers314/batoid).
+ *
+ * The reference boundary replaced here is the pybind11 module
+ * `batoid._batoid` (pysrc/batoid.cpp:31-154).  Each function below cites the
+ * reference function whose role it takes.
+ */
+#ifndef BATOID_AMD_H
+#define BATOID_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BT_ABI_VERSION 1
+
+/* ---- status codes ------------------------------------------------------ */
+#define BT_OK            0
+#define BT_E_NULL       -1   /* a required pointer is NULL                  */
+#define BT_E_SIZE       -2   /* a size or count is out of range             */
+#define BT_E_KIND       -3   /* unknown surface/medium/obscuration kind     */
+#define BT_E_PLAN       -4   /* malformed plan                              */
+#define BT_E_UNSUPPORTED -5  /* valid request this build does not implement */
+
+/* ---- ray columns --------------------------------------------------------
+ * Structure-of-arrays, FP64 except the two flag columns (1 byte each, 0/1).
+ * Replaces CPPRayVector(x_ptr, ..., failed_ptr, size)
+ * (pysrc/rayVector.cpp:31-62; include/rayVector.h:8-39).
+ * Velocities are in units of c (|v| = 1/n); t is optical path in metres.  */
+typedef struct bt_rays {
+    double* x; double* y; double* z;
+    double* vx; double* vy; double* vz;
+    double* t;
+    double* wavelength;
+    double* flux;
+    uint8_t* vignetted;
+    uint8_t* failed;
+    int64_t n;
+} bt_rays;
+
+/* ---- surfaces -----------------------------------------------------------
+ * One record per surface; a Sum record is immediately followed by its
+ * `nchild` child records (children may not be Sums).  Derived constants are
+ * filled by the host exactly as the reference constructors compute them.
+ *   PLANE        plane.cpp:13-30
+ *   SPHERE       p = {R, R*R, 1/R, 1/R/R}                 sphere.cpp:9-72
+ *   PARABOLOID   p = {R, 1/R, 1/2/R}                      paraboloid.cpp:9-71
+ *   QUADRIC      p = {R, k, R*R, 1/R/R, k+1, 1/(k+1), R/(k+1),
+ *                     R*R/(k+1)/(k+1), (k+1)/R/R}         quadric.cpp:9-77
+ *   ASPHERE      p as QUADRIC; n1 = ncoef; a0 = coefs (index 0 <-> r^2),
+ *                a1 = dz/dr coefs 2(j+1)*coefs[j]         asphere.cpp:10-85
+ *   TILTED       p = {tanx, tany}                         tilted.cpp:14-37
+ *   BICUBIC      p = {x0, y0, dx, dy}; n1 = nx, n2 = ny; flags = use_nan;
+ *                a0..a3 = z, dzdx, dzdy, d2zdxdy row-major [ny][nx]
+ *                                                         table.cpp:58-158
+ *   POLYNOMIAL   p = {xorigin, yorigin}; n1 = xsize, n2 = ysize;
+ *                a0 = coefs[ysize][xsize], a1 = gradx, a2 = grady
+ *                ([ysize-1][xsize-1])                     polynomialSurface.cpp:35-66
+ *   SUM          nchild children follow                   sum.cpp:23-56
+ */
+enum bt_surface_kind {
+    BT_PLANE = 0, BT_SPHERE = 1, BT_PARABOLOID = 2, BT_QUADRIC = 3,
+    BT_ASPHERE = 4, BT_TILTED = 5, BT_BICUBIC = 6, BT_POLYNOMIAL = 7,
+    BT_SUM = 8
+};
+#define BT_MAX_SUM_CHILDREN 8
+
+typedef struct bt_surface {
+    int32_t kind;
+    int32_t nchild;
+    int32_t n1, n2;
+    int32_t flags;
+    int32_t pad_;
+    double p[12];
+    const double* a0;
+    const double* a1;
+    const double* a2;
+    const double* a3;
+} bt_surface;
+
+/* ---- media --------------------------------------------------------------
+ *   CONST      p = {n}                                    medium.cpp:30-32
+ *   TABLE      n = size; a0 = wavelengths, a1 = indices   medium.cpp:89-105
+ *   SELLMEIER  p = {B1, B2, B3, C1, C2, C3}               medium.cpp:145-149
+ *   SUMITA     p = {A0, ..., A5}                          medium.cpp:186-191
+ *   AIR        p = {P*7.50061683, T-273.15, W*7.50061683,
+ *                   P, T, W}  (kPa, K, kPa inputs kept)   medium.cpp:229-245
+ */
+enum bt_medium_kind {
+    BT_MED_CONST = 0, BT_MED_TABLE = 1, BT_MED_SELLMEIER = 2,
+    BT_MED_SUMITA = 3, BT_MED_AIR = 4
+};
+
+typedef struct bt_medium {
+    int32_t kind;
+    int32_t n;
+    double p[6];
+    const double* a0;
+    const double* a1;
+} bt_medium;
+
+/* ---- obscurations -------------------------------------------------------
+ * An obscuration is a postfix program of `len` consecutive records; the
+ * value left on the stack is `contains(x, y)`.
+ *   CIRCLE     p = {radius, x0, y0}                       obscuration.cpp:32-34
+ *   ANNULUS    p = {inner, outer, x0, y0}                 obscuration.cpp:80-83
+ *   RECTANGLE  p = {width, height, x0, y0, cos th, sin th, th}
+ *                                                         obscuration.cpp:117-121
+ *   RAY        p = {width, x0, y0, cos th, sin th, th}    obscuration.cpp:155-159
+ *   POLYGON    n = size; a0 = xs, a1 = ys                 obscuration.cpp:201-225
+ *   NEGATION   pops 1                                     obscuration.cpp:320-322
+ *   UNION      pops n                                     obscuration.cpp:364-370
+ *   INTERSECTION pops n                                   obscuration.cpp:415-421
+ */
+enum bt_obsc_op {
+    BT_OB_CIRCLE = 0, BT_OB_ANNULUS = 1, BT_OB_RECTANGLE = 2, BT_OB_RAY = 3,
+    BT_OB_POLYGON = 4, BT_OB_NEGATION = 5, BT_OB_UNION = 6,
+    BT_OB_INTERSECTION = 7
+};
+#define BT_OBSC_MAX_STACK 32
+
+typedef struct bt_obsc {
+    int32_t op;
+    int32_t n;
+    double p[8];
+    const double* a0;
+    const double* a1;
+} bt_obsc;
+
+/* ---- plans --------------------------------------------------------------
+ * A plan is the flattened Optic.trace of a CompoundOptic
+ * (batoid/optic.py:391-424, 1013-1049): one step per Interface, each
+ * carrying the coordinate transform from the previous frame
+ * (coordTransform.py:16-20; drot row-major as numpy's ravel, trace.py:76),
+ * the interaction, the surface, the outgoing medium and the obscuration.  */
+enum bt_action { BT_ACT_INTERSECT = 0, BT_ACT_REFLECT = 1, BT_ACT_REFRACT = 2 };
+
+typedef struct bt_step {
+    int32_t action;
+    int32_t surface;    /* index of the first record in the surface table   */
+    int32_t medium;     /* index in the medium table (REFRACT), else -1     */
+    int32_t obsc;       /* index of the first obscuration record, or -1     */
+    int32_t obsc_len;   /* number of obscuration records                    */
+    int32_t pad_;
+    double dr[3];
+    double drot[9];
+} bt_step;
+
+/* bt_plan.flags */
+#define BT_PLAN_NO_REFRACT 1   /* no step refracts: wavelength is not read */
+
+typedef struct bt_plan {
+    int32_t nstep;
+    int32_t flags;
+    const bt_step* steps;         /* device */
+    const bt_surface* surfaces;   /* device */
+    const bt_medium* media;       /* device */
+    const bt_obsc* obscs;         /* device */
+} bt_plan;
+
+/* ---- library ----------------------------------------------------------- */
+int bt_abi_version(void);
+const char* bt_last_error(void);
+/* sizeof of bt_rays, bt_surface, bt_medium, bt_obsc, bt_step, bt_plan, in
+ * that order, written to sizes[0..n-1]; lets a binding check its layout. */
+int bt_struct_sizes(int64_t* sizes, int n);
+/* Device properties the host uses for launch sizing. */
+int bt_device_info(int* cu_count, int* arch_major, int* arch_minor);
+
+/* ---- coordinate transforms ---------------------------------------------
+ * Replace _batoid.applyForwardTransform / applyReverseTransform
+ * (src/batoid.cpp:71-148, pysrc/batoid.cpp:53-54) and the *Arrays forms
+ * (src/batoid.cpp:8-38, pysrc/batoid.cpp:61-102).  dr[3], drot[9] are HOST
+ * arrays (passed by value to the kernel).                                  */
+int bt_apply_forward_transform(const double* dr, const double* drot,
+                               bt_rays rays, void* stream);
+int bt_apply_reverse_transform(const double* dr, const double* drot,
+                               bt_rays rays, void* stream);
+int bt_apply_forward_transform_arrays(const double* dr, const double* drot,
+                                      double* x, double* y, double* z,
+                                      int64_t n, void* stream);
+int bt_apply_reverse_transform_arrays(const double* dr, const double* drot,
+                                      double* x, double* y, double* z,
+                                      int64_t n, void* stream);
+/* Replaces _batoid.finishParallel (src/batoid.cpp:40-69,
+ * pysrc/batoid.cpp:103-121): rays on the stop surface are moved back onto
+ * the plane through `dr` perpendicular to `vv`.                            */
+int bt_finish_parallel(const double* dr, const double* drot, const double* vv,
+                       double* x, double* y, double* z, int64_t n,
+                       void* stream);
+
+/* ---- per-surface trace --------------------------------------------------
+ * Replace _batoid.intersect / reflect / refract (src/batoid.cpp:175-506,
+ * pysrc/batoid.cpp:55-57) with coating = None and the caller's niter.
+ * `surface` is a device pointer to the surface's first record; `m2` a device
+ * pointer to the outgoing medium (m1 is not needed: the reference takes n1
+ * from |v|, src/batoid.cpp:454-458).  `obsc`/`obsc_len` optionally fold the
+ * Interface's obscuration into the same pass (optic.py:419-422); pass NULL/0
+ * for the plain reference call.                                            */
+int bt_intersect(const bt_surface* surface, const double* dr,
+                 const double* drot, bt_rays rays, int niter,
+                 const bt_obsc* obsc, int obsc_len, void* stream);
+int bt_reflect(const bt_surface* surface, const double* dr,
+               const double* drot, bt_rays rays, int niter,
+               const bt_obsc* obsc, int obsc_len, void* stream);
+int bt_refract(const bt_surface* surface, const double* dr,
+               const double* drot, const bt_medium* m2, bt_rays rays,
+               int niter, const bt_obsc* obsc, int obsc_len, void* stream);
+/* Replaces _batoid.obscure (src/batoid.cpp:151-172, pysrc/batoid.cpp:59). */
+int bt_obscure(const bt_obsc* obsc, int obsc_len, bt_rays rays, void* stream);
+
+/* Whole-optic fused trace: every step of `plan` applied to each ray while it
+ * sits in registers; reads `in` once and writes `out` once (in == out is
+ * the in-place Optic.trace).  Same per-ray results as calling the per-step
+ * functions in order.  No single reference function: replaces the loop of
+ * CompoundOptic.trace (optic.py:1013-1049) over trace.py:47-116.          */
+int bt_trace_plan(bt_plan plan, bt_rays in, bt_rays out, int niter,
+                  void* stream);
+
+/* ---- surface / medium / obscuration queries -----------------------------
+ * Replace CPPSurface.sag / .normal (pysrc/surface.cpp:11-21),
+ * CPPMedium.getN (pysrc/medium.cpp), CPPObsc*.contains
+ * (pysrc/obscuration.cpp).  All arrays are device arrays of length n;
+ * `normal_out` is [3][n] (nx block, ny block, nz block).                   */
+int bt_surface_sag(const bt_surface* surface, const double* x,
+                   const double* y, double* out, int64_t n, void* stream);
+int bt_surface_normal(const bt_surface* surface, const double* x,
+                      const double* y, double* normal_out, int64_t n,
+                      void* stream);
+int bt_medium_get_n(const bt_medium* medium, const double* wavelength,
+                    double* out, int64_t n, void* stream);
+int bt_obsc_contains(const bt_obsc* obsc, int obsc_len, const double* x,
+                     const double* y, uint8_t* out, int64_t n, void* stream);
+
+/* ---- RayVector field operations -----------------------------------------
+ * Replace CPPRayVector.positionAtTime / propagateInPlace / phase /
+ * amplitude / sumAmplitude (src/rayVector.cpp:26-222,
+ * pysrc/rayVector.cpp:63-98).  `amp_out` is interleaved complex128 [n][2].
+ * bt_sum_amplitude writes one complex128 (2 doubles) to the DEVICE pointer
+ * `out`; `workspace` must hold bt_sum_amplitude_workspace_bytes(n) bytes.  */
+int bt_position_at_time(bt_rays rays, double t, double* xout, double* yout,
+                        double* zout, void* stream);
+int bt_propagate_in_place(bt_rays rays, double t, void* stream);
+int bt_phase(bt_rays rays, double x, double y, double z, double t,
+             double* out, void* stream);
+int bt_amplitude(bt_rays rays, double x, double y, double z, double t,
+                 double* amp_out, void* stream);
+int64_t bt_sum_amplitude_workspace_bytes(int64_t n);
+int bt_sum_amplitude(bt_rays rays, double x, double y, double z, double t,
+                     int ignore_vignetted, double* out, void* workspace,
+                     void* stream);
+
+/* ---- Huygens PSF ---------------------------------------------------------
+ * The reference evaluates the image of huygensPSF (analysis.py:181-305) as a
+ * Python loop of sumAmplitude calls over every pixel (analysis.py:302-303,
+ * rayVector.cpp:169-222).  Pixel (row j, column i) sits at
+ *     p = p0 + (i - nx/2) a1 + (j - ny/2) a2,  z = 0,
+ * with a1, a2 the lattice's primitive vectors (lattice.py:6-14 and the
+ * transposition at analysis.py:296-297).  `amp_out` receives complex128
+ * amplitudes [ny][nx][2]; intensity is |amp|^2.
+ *
+ * bt_huygens_direct: FP64 direct sum, one exp(i phi) per (pixel, ray): the
+ *   reference's arithmetic on the GPU (the oracle path for the GEMM).
+ * bt_huygens_gemm: the sum factorised as a complex GEMM
+ *   A[j][i] = sum_r c_r U[i][r] V[r][j], c_r = flux e^{i phi_r(p0)},
+ *   U = e^{i m alpha_r}, V = e^{i n beta_r}, with every phase reduced mod 2pi
+ *   in FP64 before the exponentials; the GEMM runs on MFMA with
+ *   `precision` = BT_PSF_FP32 (v_mfma_f32 *_f32 operands) or BT_PSF_BF16X3
+ *   (split hi+lo bf16 operands, three products, FP32 accumulation).
+ *   Both take a device `workspace` of at least
+ *   bt_huygens_workspace_bytes(n, nx, ny) bytes for partial sums.           */
+enum bt_psf_precision { BT_PSF_FP32 = 0, BT_PSF_BF16X3 = 1 };
+
+int64_t bt_huygens_workspace_bytes(int64_t n, int nx, int ny);
+int bt_huygens_direct(bt_rays rays, const double* p0, const double* a1,
+                      const double* a2, int nx, int ny, double t,
+                      int ignore_vignetted, double* amp_out, void* workspace,
+                      int64_t workspace_bytes, void* stream);
+int bt_huygens_gemm(bt_rays rays, const double* p0, const double* a1,
+                    const double* a2, int nx, int ny, double t,
+                    int ignore_vignetted, int precision, double* amp_out,
+                    void* workspace, int64_t workspace_bytes, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BATOID_AMD_H */

@@ -1,0 +1,186 @@
+"""Host-pointer bindings of the oracle libraries (test infrastructure only)."""
+import ctypes
+import os
+
+import numpy as np
+
+from batoid_amd import _lib as L
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_PATH = os.path.join(_HERE, "liboracle.so")
+REF_PATH = os.path.join(_HERE, "_ref", "libbatoid_ref.so")
+
+_V = ctypes.c_void_p
+_D = ctypes.c_double
+_I = ctypes.c_int
+_LL = ctypes.c_int64
+_R = L.BtRays
+
+_ORACLE_SIGS = {
+    "or_sag_n": [_V, _V, _V, _V, _LL, _I],
+    "or_normal_n": [_V, _V, _V, _V, _LL, _I],
+    "or_get_n_n": [_V, _V, _V, _LL, _I],
+    "or_contains_n": [_V, _I, _V, _V, _V, _LL, _I],
+    "or_apply_forward_transform": [_V, _V, _R, _I],
+    "or_apply_reverse_transform": [_V, _V, _R, _I],
+    "or_apply_forward_transform_arrays": [_V, _V, _V, _V, _V, _LL, _I],
+    "or_apply_reverse_transform_arrays": [_V, _V, _V, _V, _V, _LL, _I],
+    "or_finish_parallel": [_V, _V, _V, _V, _V, _V, _LL],
+    "or_intersect": [_V, _V, _V, _R, _I, _I],
+    "or_reflect": [_V, _V, _V, _R, _I, _I],
+    "or_refract": [_V, _V, _V, _V, _R, _I, _I],
+    "or_obscure": [_V, _I, _R, _I],
+    "or_trace_plan": [_V, _R, _I, _I],
+    "or_position_at_time": [_R, _D, _V, _V, _V, _I],
+    "or_propagate_in_place": [_R, _D, _I],
+    "or_phase": [_R, _D, _D, _D, _D, _V, _I],
+    "or_amplitude": [_R, _D, _D, _D, _D, _V, _I],
+    "or_sum_amplitude": [_R, _D, _D, _D, _D, _I, _V, _I],
+    "or_huygens_direct": [_R, _V, _V, _V, _I, _I, _D, _I, _V, _I],
+}
+
+_REF_SIGS = {
+    "ref_intersect": [_V, _V, _V, _R, _I, _I],
+    "ref_reflect": [_V, _V, _V, _R, _I, _I],
+    "ref_refract": [_V, _V, _V, _V, _R, _I, _I],
+    "ref_obscure": [_V, _I, _R, _I],
+    "ref_sag_n": [_V, _V, _V, _V, _LL],
+    "ref_normal_n": [_V, _V, _V, _V, _LL],
+    "ref_get_n_n": [_V, _V, _V, _LL],
+    "ref_contains_n": [_V, _I, _V, _V, _V, _LL],
+    "ref_apply_forward_transform": [_V, _V, _R, _I],
+    "ref_apply_reverse_transform": [_V, _V, _R, _I],
+    "ref_finish_parallel": [_V, _V, _V, _V, _V, _V, _LL],
+    "ref_phase": [_R, _D, _D, _D, _D, _V, _I],
+    "ref_sum_amplitude": [_R, _D, _D, _D, _D, _I, _V, _I],
+}
+
+_libs = {}
+
+
+def _load(path, sigs):
+    if path in _libs:
+        return _libs[path]
+    if not os.path.exists(path):
+        raise FileNotFoundError(f"{path} not built (make -C oracle)")
+    lib = ctypes.CDLL(path)
+    for name, args in sigs.items():
+        fn = getattr(lib, name)
+        fn.restype = None
+        fn.argtypes = args
+    _libs[path] = lib
+    return lib
+
+
+def load():
+    return _load(ORACLE_PATH, _ORACLE_SIGS)
+
+
+def ref_available():
+    return os.path.exists(REF_PATH)
+
+
+def load_ref():
+    return _load(REF_PATH, _REF_SIGS)
+
+
+def nthreads():
+    """Threads for the oracle's OpenMP loops: ORACLE_THREADS, else
+    OMP_NUM_THREADS, else the CPU count capped at 16."""
+    for var in ("ORACLE_THREADS", "OMP_NUM_THREADS"):
+        if os.environ.get(var):
+            return max(1, int(os.environ[var]))
+    return max(1, min(os.cpu_count() or 1, 16))
+
+
+_FCOLS = ("x", "y", "z", "vx", "vy", "vz", "t", "wavelength", "flux")
+_BCOLS = ("vignetted", "failed")
+
+
+class HostRays:
+    """numpy ray columns with a bt_rays view of host pointers."""
+
+    def __init__(self, **cols):
+        n = len(cols["x"])
+        self.cols = {}
+        for k in _FCOLS:
+            self.cols[k] = np.ascontiguousarray(
+                np.broadcast_to(np.asarray(cols.get(k, 0.0 if k != "flux" else 1.0), float), (n,)).copy())
+        for k in _BCOLS:
+            self.cols[k] = np.ascontiguousarray(
+                np.broadcast_to(np.asarray(cols.get(k, False), bool), (n,)).astype(np.uint8))
+
+    @classmethod
+    def from_rv(cls, rv):
+        """Host copy of a batoid_amd.RayVector's columns."""
+        return cls(**{k: getattr(rv, k) for k in _FCOLS + _BCOLS})
+
+    def copy(self):
+        h = HostRays.__new__(HostRays)
+        h.cols = {k: v.copy() for k, v in self.cols.items()}
+        return h
+
+    def __len__(self):
+        return len(self.cols["x"])
+
+    def __getitem__(self, k):
+        return self.cols[k]
+
+    @property
+    def struct(self):
+        r = L.BtRays()
+        for k in _FCOLS + _BCOLS:
+            setattr(r, k, self.cols[k].ctypes.data)
+        r.n = len(self)
+        return r
+
+
+class _HostTable:
+    def __init__(self, records, arrays):
+        self.keep = []
+        recs = []
+        for rec, arrs in zip(records, arrays):
+            rec = type(rec).from_buffer_copy(rec)
+            for k, a in enumerate(arrs):
+                if a is None:
+                    continue
+                a = np.ascontiguousarray(a, dtype=float)
+                self.keep.append(a)
+                setattr(rec, f"a{k}", a.ctypes.data)
+            recs.append(rec)
+        self.count = len(recs)
+        self.array = (type(records[0]) * len(recs))(*recs) if recs else None
+
+    def ptr(self, i=0):
+        if self.array is None:
+            return None
+        return ctypes.addressof(self.array) + i * ctypes.sizeof(self.array._type_)
+
+
+def host_records(obj):
+    """Host-pointer table of a batoid_amd Surface / Medium / Obscuration."""
+    recs, arrs = obj._records()
+    return _HostTable(recs, arrs)
+
+
+class HostPlan:
+    """A batoid_amd.plan.PlanSpec with host pointers, for or_trace_plan."""
+
+    def __init__(self, spec):
+        self.surfaces = _HostTable(spec.surf_recs, spec.surf_arrs)
+        self.media = _HostTable(spec.med_recs, spec.med_arrs) if spec.med_recs else None
+        self.obscs = _HostTable(spec.ob_recs, spec.ob_arrs) if spec.ob_recs else None
+        self.steps = (L.BtStep * len(spec.steps))(*spec.steps)
+        p = L.BtPlan()
+        p.nstep = len(spec.steps)
+        p.flags = spec.flags
+        p.steps = ctypes.addressof(self.steps)
+        p.surfaces = self.surfaces.ptr()
+        p.media = self.media.ptr() if self.media else None
+        p.obscs = self.obscs.ptr() if self.obscs else None
+        self.struct = p
+
+    def trace(self, rays, niter=5, threads=None):
+        load().or_trace_plan(ctypes.byref(self.struct), rays.struct, niter,
+                             threads or nthreads())
+        return rays

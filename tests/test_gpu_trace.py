@@ -1,0 +1,223 @@
+"""GPU parity: every trace kernel of libbatoid_amd.so against the oracle on
+the same seeded inputs.  The trace path is FP64 with no contraction on
+either side, so the bar is bit-for-bit identity of every column, NaNs in
+the same places and flags equal."""
+import ctypes
+
+import numpy as np
+import pytest
+import torch
+
+import batoid_amd as bt
+import oracle
+from oracle.host import HostRays, HostPlan, host_records, nthreads
+from batoid_amd.plan import PlanSpec, collect_steps
+from helpers import (surfaces, media, obscurations, random_rays, transforms,
+                     assert_cols_identical, FCOLS, BCOLS)
+
+pytestmark = pytest.mark.gpu
+
+N = 20000
+NITER = 5
+
+
+def _dv(a):
+    a = np.ascontiguousarray(a, dtype=float).ravel()
+    return (ctypes.c_double * len(a))(*a)
+
+
+def _gpu_rays(h):
+    return bt.RayVector(**{k: h[k] for k in FCOLS}, vignetted=h["vignetted"].astype(bool),
+                        failed=h["failed"].astype(bool))
+
+
+def _host(rv):
+    torch.cuda.synchronize()
+    return {k: getattr(rv, k) for k in FCOLS + BCOLS}
+
+
+def _frame(dr, drot):
+    """A CoordSys whose transform from the global frame is (dr, drot)."""
+    return bt.CoordSys(origin=dr, rot=drot)
+
+
+@pytest.mark.parametrize("name,surf", surfaces(), ids=[s[0] for s in surfaces()])
+@pytest.mark.parametrize("action", ["intersect", "reflect", "refract"])
+def test_surface_kernels_bit_exact(name, surf, action):
+    print(f"case {name}/{action}", flush=True)
+    O = oracle.load()
+    tab = host_records(surf)
+    silica = media()[2][1]
+    mtab = host_records(silica)
+    for tname, (dr, drot) in transforms():
+        for seed, upward in ((1, 0.0), (2, 0.3)):
+            base = HostRays(**random_rays(N, seed=seed, upward=upward, spread=0.3))
+            ref = base.copy()
+            args = (tab.ptr(), _dv(dr), _dv(drot))
+            if action == "refract":
+                O.or_refract(*args, mtab.ptr(), ref.struct, NITER, nthreads())
+            else:
+                getattr(O, f"or_{action}")(*args, ref.struct, NITER, nthreads())
+            rv = _gpu_rays(base)
+            cs = _frame(dr, drot)
+            if action == "refract":
+                bt.refract(surf, rv, bt.vacuum, silica, coordSys=cs)
+            else:
+                getattr(bt, action)(surf, rv, coordSys=cs)
+            assert_cols_identical(_host(rv), ref.cols, what=f"{name}/{action}/{tname}/{seed}")
+
+
+@pytest.mark.parametrize("mname,med", media(), ids=[m[0] for m in media()])
+def test_refract_media_bit_exact(mname, med):
+    O = oracle.load()
+    surf = bt.Sphere(2.3)
+    tab, mtab = host_records(surf), host_records(med)
+    base = HostRays(**random_rays(N, seed=9, n_index=1.0, spread=0.4))
+    ref = base.copy()
+    O.or_refract(tab.ptr(), _dv(np.zeros(3)), _dv(np.eye(3)), mtab.ptr(), ref.struct,
+                 NITER, nthreads())
+    rv = _gpu_rays(base)
+    bt.refract(surf, rv, bt.vacuum, med)
+    assert_cols_identical(_host(rv), ref.cols, what=mname)
+
+
+def test_total_internal_reflection_bit_exact():
+    """Rays leaving glass at grazing angles: NaN velocities and both flags
+    (batoid.cpp:478-487)."""
+    O = oracle.load()
+    surf = bt.Plane()
+    med = bt.ConstMedium(1.0)
+    base = HostRays(**random_rays(N, seed=10, n_index=1.5, spread=1.0))
+    ref = base.copy()
+    tab, mtab = host_records(surf), host_records(med)
+    O.or_refract(tab.ptr(), _dv(np.zeros(3)), _dv(np.eye(3)), mtab.ptr(), ref.struct,
+                 NITER, nthreads())
+    tir = np.isnan(ref["vx"]).sum()
+    assert tir > 100
+    rv = _gpu_rays(base)
+    bt.refract(surf, rv, bt.ConstMedium(1.5), med)
+    assert_cols_identical(_host(rv), ref.cols, what="TIR")
+
+
+@pytest.mark.parametrize("oname,obsc", obscurations(), ids=[o[0] for o in obscurations()])
+def test_obscurations_bit_exact(oname, obsc):
+    O = oracle.load()
+    tab = host_records(obsc)
+    base = HostRays(**random_rays(N, seed=12))
+    ref = base.copy()
+    O.or_obscure(tab.ptr(), tab.count, ref.struct, nthreads())
+    rv = _gpu_rays(base)
+    obsc.obscure(rv)
+    assert_cols_identical(_host(rv), ref.cols, what=oname)
+    rng = np.random.default_rng(13)
+    x, y = rng.uniform(-1, 1, N), rng.uniform(-1, 1, N)
+    oc = np.empty(N, np.uint8)
+    O.or_contains_n(tab.ptr(), tab.count, x.ctypes.data, y.ctypes.data, oc.ctypes.data, N,
+                    nthreads())
+    np.testing.assert_array_equal(obsc.contains(x, y), oc.astype(bool))
+
+
+@pytest.mark.parametrize("name,surf", surfaces(), ids=[s[0] for s in surfaces()])
+def test_sag_normal_bit_exact(name, surf):
+    O = oracle.load()
+    tab = host_records(surf)
+    rng = np.random.default_rng(14)
+    x, y = rng.uniform(-1.5, 1.5, N), rng.uniform(-1.5, 1.5, N)
+    s = np.empty(N)
+    nrm = np.empty(3 * N)
+    O.or_sag_n(tab.ptr(), x.ctypes.data, y.ctypes.data, s.ctypes.data, N, nthreads())
+    O.or_normal_n(tab.ptr(), x.ctypes.data, y.ctypes.data, nrm.ctypes.data, N, nthreads())
+    assert_cols_identical({"s": surf.sag(x, y)}, {"s": s}, cols=("s",), what=name)
+    g = surf.normal(x, y)
+    assert_cols_identical({"n": g.T.ravel()}, {"n": nrm}, cols=("n",), what=name)
+
+
+@pytest.mark.parametrize("mname,med", media(), ids=[m[0] for m in media()])
+def test_media_bit_exact(mname, med):
+    O = oracle.load()
+    tab = host_records(med)
+    w = np.random.default_rng(15).uniform(250e-9, 1200e-9, N)
+    o = np.empty(N)
+    O.or_get_n_n(tab.ptr(), w.ctypes.data, o.ctypes.data, N, nthreads())
+    assert_cols_identical({"n": med.getN(w)}, {"n": o}, cols=("n",), what=mname)
+
+
+def test_transforms_and_propagation_bit_exact():
+    O = oracle.load()
+    for tname, (dr, drot) in transforms():
+        base = HostRays(**random_rays(N, seed=16))
+        ref = base.copy()
+        O.or_apply_forward_transform(_dv(dr), _dv(drot), ref.struct, nthreads())
+        rv = _gpu_rays(base)
+        rv.toCoordSys(_frame(dr, drot))
+        assert_cols_identical(_host(rv), ref.cols, what=f"forward {tname}")
+        O.or_apply_reverse_transform(_dv(dr), _dv(drot), ref.struct, nthreads())
+        bt.CoordTransform(bt.CoordSys(), _frame(dr, drot)).applyReverse(rv)
+        assert_cols_identical(_host(rv), ref.cols, what=f"reverse {tname}")
+    base = HostRays(**random_rays(N, seed=17))
+    ref = base.copy()
+    O.or_propagate_in_place(ref.struct, 3.25, nthreads())
+    rv = _gpu_rays(base)
+    pos = rv.positionAtTime(3.25)
+    rv.propagate(3.25)
+    assert_cols_identical(_host(rv), ref.cols, what="propagate")
+    assert_cols_identical({"x": pos[:, 0], "y": pos[:, 1], "z": pos[:, 2]}, ref.cols,
+                          cols=("x", "y", "z"), what="positionAtTime")
+
+
+def test_field_operations():
+    O = oracle.load()
+    base = HostRays(**random_rays(N, seed=18))
+    rv = _gpu_rays(base)
+    p = np.empty(N)
+    O.or_phase(base.struct, 0.1, -0.2, 0.3, 1.5, p.ctypes.data, nthreads())
+    assert_cols_identical({"p": rv.phase([0.1, -0.2, 0.3], 1.5)}, {"p": p}, cols=("p",))
+    a = np.empty(2 * N)
+    O.or_amplitude(base.struct, 0.1, -0.2, 0.3, 1.5, a.ctypes.data, nthreads())
+    # exp(i phi): FP64 sin/cos of the device math library vs libm, <= 1 ulp
+    np.testing.assert_allclose(rv.amplitude([0.1, -0.2, 0.3], 1.5).view(np.float64), a,
+                               rtol=0, atol=4e-16 * np.max(np.abs(p)))
+    for ign in (True, False):
+        s = np.empty(2)
+        O.or_sum_amplitude(base.struct, 0.1, -0.2, 0.3, 1.5, int(ign), s.ctypes.data, 1)
+        g = rv.sumAmplitude([0.1, -0.2, 0.3], 1.5, ignoreVignetted=ign)
+        assert abs(g - complex(s[0], s[1])) <= 1e-12 * N
+
+
+def test_fused_plan_matches_per_surface_and_oracle():
+    """Whole-optic fused kernel == oracle's per-Interface passes, on an optic
+    that mixes every interaction, media, obscurations and a perturbed Sum."""
+    from helpers import zernike_surface, bicubic_surface
+    silica = bt.SellmeierMedium(0.6961663, 0.4079426, 0.8974794, 0.00467914825849,
+                                0.013512063073959999, 97.93400253792099)
+    cs = bt.CoordSys()
+    m1 = bt.Mirror(bt.Sum(bt.Asphere(19.835, -1.215, [0.0, -1.381e-9]),
+                          zernike_surface(jmax=15, R_outer=1.0), bicubic_surface()),
+                   name="M1", coordSys=cs, obscuration=bt.ObscNegation(bt.ObscAnnulus(0.1, 1.0)))
+    l1 = bt.Lens([
+        bt.RefractiveInterface(bt.Sphere(-2.0), name="L1a", coordSys=cs.shiftLocal([0, 0, 5.0]),
+                               inMedium=bt.vacuum, outMedium=silica,
+                               obscuration=bt.ObscNegation(bt.ObscCircle(0.7))),
+        bt.RefractiveInterface(bt.Quadric(-3.0, -0.5), name="L1b",
+                               coordSys=cs.shiftLocal([0.001, 0, 5.05]).rotateLocal(bt.RotX(0.002)),
+                               inMedium=silica, outMedium=bt.vacuum),
+    ], name="L1", inMedium=bt.vacuum, outMedium=bt.vacuum)
+    baffle = bt.Baffle(bt.Plane(), name="B", coordSys=cs.shiftLocal([0, 0, 6.0]),
+                       obscuration=bt.ObscRectangle(0.05, 3.0))
+    det = bt.Detector(bt.Tilted(0.001, 0.0), name="D", coordSys=cs.shiftLocal([0, 0, 9.0]))
+    optic = bt.CompoundOptic([m1, l1, baffle, det], name="test", inMedium=bt.vacuum)
+
+    base = HostRays(**random_rays(N, seed=19, z0=2.0, spread=0.02, radius=1.0))
+    spec = PlanSpec(collect_steps(optic), bt.CoordSys())
+    ref = HostPlan(spec).trace(base.copy(), niter=NITER)
+    assert 0 < ref["vignetted"].sum() < N
+
+    rv = _gpu_rays(base)
+    optic.trace(rv)
+    assert rv.coordSys == det.coordSys
+    assert_cols_identical(_host(rv), ref.cols, what="fused plan")
+
+    rv2 = _gpu_rays(base)
+    for item in [m1, l1.items[0], l1.items[1], baffle, det]:
+        item.trace(rv2)
+    assert_cols_identical(_host(rv2), ref.cols, what="per-surface")
