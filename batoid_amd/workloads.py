@@ -1,0 +1,113 @@
+"""Benchmark optics and ray bundles.
+
+* ``newtonian()``: one Paraboloid mirror (R = 10 m in batoid's frame, where
+  rays arrive travelling -z; 2 m aperture, 0.4 m central obscuration) and a
+  flat Detector at its focus z = R/2, in vacuum.
+* ``tma()``: a 3-mirror anastigmat with three fused-silica lenses (Sellmeier),
+  a filter and a detector -- the LSST r-band prescription
+  (batoid/data/LSST/LSST_r.yaml:17-188) with the primary's even asphere
+  extended to five terms.  12 interfaces.
+* ``tma_rays()``: the hexapolar pupil of the TMA: asPolar(outer=4.18,
+  inner=2.558, nrad=1120, naz=11080) = 9,998,544 rays.
+* ``tma_fields()``: 189 field points uniform in a 1.75 degree disk.
+"""
+import numpy as np
+
+from .parse import parse_optic
+from .rayVector import RayVector
+
+WAVELENGTHS = (350e-9, 490e-9, 630e-9, 770e-9, 910e-9, 1050e-9)
+
+_SILICA = {'type': 'SellmeierMedium', 'B1': 0.696183020, 'B2': 0.407925877,
+           'B3': 0.897464057, 'C1': 0.00467926519, 'C2': 0.0135122244,
+           'C3': 97.9323636}
+
+
+def _lens(name, z, front, back, r_clear, dz):
+    return {'type': 'Lens', 'name': name, 'medium': _SILICA, 'coordSys': {'z': z},
+            'items': [
+                {'type': 'RefractiveInterface', 'name': name + '_entrance', 'surface': front,
+                 'obscuration': {'type': 'ClearCircle', 'radius': r_clear}},
+                {'type': 'RefractiveInterface', 'name': name + '_exit', 'surface': back,
+                 'obscuration': {'type': 'ClearCircle', 'radius': r_clear},
+                 'coordSys': {'z': dz}}]}
+
+
+def tma_config(m1_coefs=(0.0, -1.381e-9, -4.3e-20, 2.1e-25, -1.5e-30)):
+    return {
+        'type': 'CompoundOptic', 'name': 'TMA', 'inMedium': 1.0,
+        'backDist': 15.0, 'sphereRadius': 5.0, 'pupilSize': 8.36,
+        'pupilObscuration': 0.612,
+        'stopSurface': {'type': 'Interface', 'name': 'entrancePupil',
+                        'surface': {'type': 'Plane'}, 'coordSys': {'z': 0.4393899}},
+        'items': [
+            {'type': 'Mirror', 'name': 'M1',
+             'surface': {'type': 'Asphere', 'R': 19.835, 'conic': -1.215,
+                         'coefs': list(m1_coefs)},
+             'obscuration': {'type': 'ClearAnnulus', 'outer': 4.18, 'inner': 2.558}},
+            {'type': 'Mirror', 'name': 'M2',
+             'surface': {'type': 'Asphere', 'R': 6.788, 'conic': -0.222,
+                         'coefs': [0.0, 1.274e-5, 9.68e-7]},
+             'obscuration': {'type': 'ClearAnnulus', 'outer': 1.71, 'inner': 0.9},
+             'coordSys': {'z': 6.1562006}},
+            {'type': 'Mirror', 'name': 'M3',
+             'surface': {'type': 'Asphere', 'R': 8.3445, 'conic': 0.155,
+                         'coefs': [0.0, 4.5e-7, 8.15e-9]},
+             'obscuration': {'type': 'ClearAnnulus', 'outer': 2.508, 'inner': 0.55},
+             'coordSys': {'z': -0.2338}},
+            {'type': 'CompoundOptic', 'name': 'Camera',
+             'coordSys': {'z': 3.3974725882045593},
+             'items': [
+                 _lens('L1', 0.0, {'type': 'Sphere', 'R': 2.824},
+                       {'type': 'Sphere', 'R': 5.021}, 0.775, 0.08223),
+                 _lens('L2', 0.49487202, {'type': 'Plane'},
+                       {'type': 'Asphere', 'R': 2.529, 'conic': -1.57,
+                        'coefs': [0.0, -0.001656]}, 0.551, 0.03),
+                 _lens('Filter', 0.87445202, {'type': 'Sphere', 'R': 5.632},
+                       {'type': 'Sphere', 'R': 5.606}, 0.375, 0.0179),
+                 _lens('L3', 0.94345202, {'type': 'Quadric', 'R': 3.169, 'conic': -0.962},
+                       {'type': 'Sphere', 'R': -13.36}, 0.361, 0.06),
+                 {'type': 'Detector', 'name': 'Detector', 'surface': {'type': 'Plane'},
+                  'obscuration': {'type': 'ClearCircle', 'radius': 0.4},
+                  'coordSys': {'z': 1.03195202}},
+             ]},
+        ]}
+
+
+def tma(**kwargs):
+    return parse_optic(tma_config(**kwargs))
+
+
+def tma_fields(n=189, seed=5772156, radius_deg=1.75):
+    """(theta_x, theta_y) in radians, uniform in a disk."""
+    rng = np.random.default_rng(seed)
+    r = np.deg2rad(radius_deg) * np.sqrt(rng.uniform(size=n))
+    th = rng.uniform(0, 2 * np.pi, size=n)
+    return r * np.cos(th), r * np.sin(th)
+
+
+def tma_rays(optic, theta_x, theta_y, wavelength, nrad=1120, naz=11080):
+    return RayVector.asPolar(optic=optic, outer=4.18, inner=2.558, nrad=nrad, naz=naz,
+                             theta_x=theta_x, theta_y=theta_y, wavelength=wavelength)
+
+
+def newtonian_config():
+    return {
+        'type': 'CompoundOptic', 'name': 'Newtonian', 'inMedium': 1.0,
+        'backDist': 40.0, 'pupilSize': 2.0, 'pupilObscuration': 0.2,
+        'items': [
+            {'type': 'Mirror', 'name': 'Primary', 'surface': {'type': 'Paraboloid', 'R': 10.0},
+             'obscuration': {'type': 'ClearAnnulus', 'inner': 0.2, 'outer': 1.0}},
+            {'type': 'Detector', 'name': 'Detector', 'surface': {'type': 'Plane'},
+             'coordSys': {'z': 5.0}},
+        ]}
+
+
+def newtonian():
+    return parse_optic(newtonian_config())
+
+
+def newtonian_rays(optic, n=1_000_000, seed=57721, wavelength=500e-9, theta_deg=0.1):
+    return RayVector.asPolar(optic=optic, outer=1.0, inner=0.0, nrandom=n,
+                             rng=np.random.default_rng(seed), wavelength=wavelength,
+                             theta_x=np.deg2rad(theta_deg), theta_y=0.0)

@@ -1,0 +1,213 @@
+"""Benchmark: FP64 ray-surface intersections/s of the fused whole-optic trace
+on one MI355X, with its HBM roofline and the reference CPU path beside it.
+
+Workload (one step): the 12-interface TMA + 3-lens telescope
+(batoid_amd.workloads.tma) traced over a 9,998,544-ray hexapolar pupil at
+one (field, wavelength); successive steps cycle through 6 pre-generated
+(field, wavelength) batches that are resident in HBM before timing starts.
+Each step reads its batch and writes a separate output buffer, so inputs stay
+pristine from step to step (the same per-ray traffic as the in-place
+Optic.trace).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+
+Prints one JSON line on stdout; progress goes to stderr.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+METRIC = "ray-surface intersections/s (FP64, ≤1e-12 m) + Huygens PSF TFLOP/s vs plain"
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+# Bytes the fused trace moves per ray: reads x, y, z, vx, vy, vz, t,
+# wavelength (8 B each) + 2 flag bytes; writes x..t + 2 flag bytes.
+BYTES_PER_RAY = 8 * 8 + 2 + 7 * 8 + 2
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(optic, rv, target_s=10.0):
+    """Time the reference's CPU trace (its own C++ core, oracle/_ref, one call
+    per Interface exactly as batoid's Optic.trace issues them) -- or the C
+    restatement when _ref is absent -- on a strided sample of the batch."""
+    import oracle
+    from oracle.host import HostRays, host_records, nthreads
+    from batoid_amd.plan import PlanSpec, collect_steps
+    threads = nthreads()
+    use_ref = oracle.ref_available()
+    lib = oracle.load_ref() if use_ref else oracle.load()
+    spec = PlanSpec(collect_steps(optic), rv.coordSys)
+    tabs = [host_records(i.surface) for i in spec.interfaces]
+    obscs = [host_records(i.obscuration) if i.obscuration is not None else None
+             for i in spec.interfaces]
+    meds = {}
+    for st in spec.steps:
+        if st.medium >= 0:
+            from oracle.host import _HostTable
+            meds[st.medium] = _HostTable([spec.med_recs[st.medium]], [spec.med_arrs[st.medium]])
+    full = HostRays.from_rv(rv)
+    pre = "ref_" if use_ref else "or_"
+
+    def run(n):
+        idx = np.linspace(0, len(full) - 1, n).astype(np.int64)
+        h = HostRays(**{k: v[idx] for k, v in full.cols.items()})
+        t0 = time.perf_counter()
+        for st, tab, ob in zip(spec.steps, tabs, obscs):
+            dr = (ctypes.c_double * 3)(*st.dr)
+            drot = (ctypes.c_double * 9)(*st.drot)
+            if st.action == 2:
+                getattr(lib, pre + "refract")(tab.ptr(), dr, drot, meds[st.medium].ptr(),
+                                              h.struct, 5, threads)
+            else:
+                name = "intersect" if st.action == 0 else "reflect"
+                getattr(lib, pre + name)(tab.ptr(), dr, drot, h.struct, 5, threads)
+            if ob is not None:
+                getattr(lib, pre + "obscure")(ob.ptr(), ob.count, h.struct, threads)
+        return time.perf_counter() - t0
+
+    t_small = run(20000)
+    n = int(min(len(full), max(20000, 20000 * target_s / max(t_small, 1e-6))))
+    t = run(n)
+    value = n * len(spec.steps) / t
+    return {"value": value, "unit": "ray-surface intersections/s", "cores": threads,
+            "kind": "reference" if use_ref else "port",
+            "sample": f"{n} of {len(full)} rays (strided) of the first batch, "
+                      f"{len(spec.steps)} interfaces, {t:.2f} s, "
+                      f"{'reference C++ core (oracle/_ref)' if use_ref else 'C restatement (oracle)'} "
+                      f"with {threads} OpenMP threads"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--batches", type=int, default=6)
+    ap.add_argument("--nrad", type=int, default=1120)
+    ap.add_argument("--naz", type=int, default=11080)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(--gpus must be 1)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import batoid_amd as bt
+    from batoid_amd import _lib, _device, workloads
+    from batoid_amd.global_vars import _batoid_niter
+
+    dev = _device.device()
+    optic = workloads.tma()
+    thx, thy = workloads.tma_fields()
+    batches = []
+    for b in range(args.batches):
+        wl = workloads.WAVELENGTHS[b % len(workloads.WAVELENGTHS)]
+        rv = workloads.tma_rays(optic, thx[b], thy[b], wl, nrad=args.nrad, naz=args.naz)
+        batches.append(rv)
+        log(f"batch {b}: field ({np.rad2deg(thx[b]):.3f}, {np.rad2deg(thy[b]):.3f}) deg, "
+            f"{wl*1e9:.0f} nm, {len(rv)} rays")
+    n = len(batches[0])
+    plan = optic._plan(batches[0].coordSys, False)
+    nsteps = plan.struct.nstep
+    # one output buffer; wavelength and flux are shared with each input
+    out_cols = {k: torch.empty(n, dtype=torch.float64, device=dev)
+                for k in ("x", "y", "z", "vx", "vy", "vz", "t")}
+    out_cols.update({k: torch.empty(n, dtype=torch.uint8, device=dev)
+                     for k in ("vignetted", "failed")})
+    structs = []
+    for rv in batches:
+        o = _lib.BtRays()
+        for k, v in out_cols.items():
+            setattr(o, k, v.data_ptr())
+        o.wavelength = rv.torch("wavelength").data_ptr()
+        o.flux = rv.torch("flux").data_ptr()
+        o.n = n
+        structs.append((rv._rays, o))
+    lib = _lib.load()
+    stream = _device.stream()
+
+    def step(i):
+        rin, rout = structs[i % len(structs)]
+        rc = lib.bt_trace_plan(plan.struct, rin, rout, _batoid_niter, stream)
+        if rc:
+            _lib.check(rc, "bt_trace_plan")
+
+    for i in range(args.warmup):
+        step(i)
+    torch.cuda.synchronize()
+    log(f"warmup done ({args.warmup} steps)")
+
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        starts[i].record()
+        step(i)
+        ends[i].record()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    elapsed = t1 - t0
+    kernel_ms = float(np.mean([s.elapsed_time(e) for s, e in zip(starts, ends)]))
+    value = n * nsteps * args.steps / elapsed
+    log(f"timed {args.steps} steps: {elapsed*1e3/args.steps:.3f} ms/step, "
+        f"kernel {kernel_ms:.3f} ms, {value:.3e} ray-surfaces/s")
+
+    # sanity: the traced output is a real trace (finite positions for the
+    # rays that reached the detector)
+    vig = out_cols["vignetted"].cpu().numpy().astype(bool)
+    frac_unvig = float(1.0 - vig.mean())
+
+    bytes_per_launch = BYTES_PER_RAY * n
+    achieved = bytes_per_launch / (kernel_ms * 1e-3) / 1e9
+    result = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "ray-surface intersections/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "TMA + 3 fused-silica lenses + filter + detector (12 interfaces), "
+                        "hexapolar pupil, one (field, wavelength) per step",
+            "rays": n, "interfaces": nsteps, "batches": len(batches),
+            "wavelengths_nm": [round(w * 1e9) for w in workloads.WAVELENGTHS[:len(batches)]],
+            "fields": "first 6 of 189 uniform in a 1.75 deg disk",
+            "niter": _batoid_niter, "unvignetted_fraction": round(frac_unvig, 4),
+        },
+        "roofline": {
+            "kernel": "k_plan (bt_trace_plan)",
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "bytes_per_launch": bytes_per_launch,
+            "kernel_ms": kernel_ms,
+        },
+    }
+    if not args.no_cpu:
+        log("timing the CPU baseline ...")
+        result["cpu_baseline"] = cpu_baseline(optic, batches[0])
+        log(f"cpu baseline {result['cpu_baseline']['value']:.3e} ray-surfaces/s")
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()
