@@ -1,13 +1,349 @@
-// bt_huygens.hip -- Huygens PSF as a complex GEMM on MFMA (placeholder for
-// the MFMA kernels; the FP64 direct sum lives in bt_field.hip).
+// bt_huygens.hip -- the Huygens PSF sum as a complex GEMM on MFMA.
+//
+// The reference evaluates, for every pixel p of the image lattice,
+//   A(p) = sum_r flux_r exp(i phi_r(p)),
+//   phi_r(p) = 2pi/lambda_r (((p - x_r).v_r)/|v_r|^2 - (t - t_r))
+// (rayVector.cpp:169-222, one sumAmplitude per pixel at analysis.py:302-303).
+// The phase is affine in p, and the pixels form a lattice
+//   p(i, j) = p0 + ci a1 + cj a2,  ci = i - nx/2, cj = j - ny/2,
+// so  phi_r(p) = phi_r(p0) + ci alpha_r + cj beta_r  and
+//   A[j][i] = sum_r  c_r e^{i ci alpha_r} e^{i cj beta_r},  c_r = flux e^{i phi_r(p0)}
+// is a complex GEMM  U[i][r] V[r][j]  with M = nx, N = ny, K = rays.
+//
+// Precision plan:
+//  * per ray, in FP64: phi_r(p0) with the reference's formula, reduced to
+//    revolutions mod 1; alpha, beta in revolutions per pixel step, each split
+//    into hi (few enough significant bits that ci * hi is exact in FP32) and
+//    lo parts;
+//  * per (pixel, ray), in FP32: theta = fract(ci*hi) + ci*lo + phi0, then
+//    sin(2 pi theta) on the transcendental unit -- no FP32 rounding of a large
+//    phase ever happens;
+//  * the GEMM runs on MFMA: v_mfma_f32_32x32x2_f32 (FP32 operands), or the
+//    split-BF16 form x = hi + lo with three v_mfma_f32_32x32x16_bf16
+//    products (hi*hi + hi*lo + lo*hi), FP32 accumulation either way;
+//  * partial images of the K splits are summed in FP64 in a fixed order.
+//
+// Complex -> real: the output rows are (i, re|im) pairs and the K index
+// (ray, re|im), with A'[2i][2r] = Re U, A'[2i][2r+1] = -Im U,
+// A'[2i+1][2r] = Im U, A'[2i+1][2r+1] = Re U, B'[2r][j] = Re V,
+// B'[2r+1][j] = Im V.  Operands are generated directly in the MFMA lane
+// layout; each wave owns a 64 (i) x 64 (j) complex output tile (eight
+// 32 x 32 accumulator blocks) and a contiguous range of rays.
 #include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+
+#include "bt_device.h"
 #include "bt_common.h"
 
 extern "C" int64_t bt_huygens_direct_workspace_bytes(int64_t n, int nx, int ny);
 
+namespace bt {
+
+typedef float v16f __attribute__((ext_vector_type(16)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+
+constexpr double kTwoPi = 2 * 3.14159265358979323846;
+constexpr int kTile = 128;          // workgroup tile (complex) in i and j
+constexpr int kRayStride = 8;       // floats per prepared ray
+constexpr int kRayGroup = 8;        // rays per bf16 MFMA step; splits align to it
+constexpr int kScanThreads = 1024;
+
+struct PsfGeom {
+    double p0x, p0y, a1x, a1y, a2x, a2y, t;
+    int nx, ny;
+    int keep_a, keep_b;             // significant bits kept in the hi parts
+};
+
+// Keep the leading `keep` significant bits of x (1 <= keep <= 24).
+__device__ __forceinline__ float hi_bits(float x, int keep)
+{
+    const uint32_t mask = ~((1u << (24 - keep)) - 1u);
+    return __uint_as_float(__float_as_uint(x) & mask);
+}
+
+__device__ __forceinline__ bool psf_valid(const bt_rays& rv, int64_t i, int ignore_vig)
+{
+    return !rv.failed[i] && !(ignore_vig && rv.vignetted[i]);
+}
+
+__global__ __launch_bounds__(kBlock) void k_psf_count(bt_rays rv, int ignore_vig,
+                                                       int* __restrict__ counts)
+{
+    __shared__ int s[kBlock / 64];
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const bool v = i < rv.n && psf_valid(rv, i, ignore_vig);
+    const uint64_t m = __ballot(v);
+    if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = __popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int c = 0;
+        for (int w = 0; w < kBlock / 64; w++) c += s[w];
+        counts[blockIdx.x] = c;
+    }
+}
+
+// Exclusive scan of the per-block counts; offsets[nb] = total.
+__global__ __launch_bounds__(kScanThreads) void k_psf_scan(const int* __restrict__ counts,
+                                                           int nb, int* __restrict__ offsets)
+{
+    __shared__ int s[kScanThreads];
+    const int per = (nb + kScanThreads - 1) / kScanThreads;
+    const int b0 = threadIdx.x * per;
+    int sum = 0;
+    for (int b = b0; b < b0 + per && b < nb; b++) sum += counts[b];
+    s[threadIdx.x] = sum;
+    __syncthreads();
+    for (int d = 1; d < kScanThreads; d <<= 1) {
+        int v = (threadIdx.x >= d) ? s[threadIdx.x - d] : 0;
+        __syncthreads();
+        s[threadIdx.x] += v;
+        __syncthreads();
+    }
+    int run = s[threadIdx.x] - sum;
+    for (int b = b0; b < b0 + per && b < nb; b++) {
+        offsets[b] = run;
+        run += counts[b];
+    }
+    if (threadIdx.x == kScanThreads - 1) offsets[nb] = s[kScanThreads - 1];
+}
+
+// Per ray: phase at p0 and per-pixel phase steps, written compacted in ray
+// order: [alpha_hi, alpha_lo, beta_hi, beta_lo, phi0_rev, flux, 0, 0].
+__global__ __launch_bounds__(kBlock) void k_psf_prep(bt_rays rv, PsfGeom g, int ignore_vig,
+                                                      const int* __restrict__ offsets,
+                                                      float* __restrict__ params)
+{
+    __shared__ int s[kBlock / 64];
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const bool v = i < rv.n && psf_valid(rv, i, ignore_vig);
+    const uint64_t m = __ballot(v);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) s[w] = __popcll(m);
+    __syncthreads();
+    if (!v) return;
+    int base = offsets[blockIdx.x];
+    for (int k = 0; k < w; k++) base += s[k];
+    const int rank = base + __popcll(m & ((1ull << lane) - 1ull));
+
+    const double x = rv.x[i], y = rv.y[i], z = rv.z[i];
+    const double vx = rv.vx[i], vy = rv.vy[i], vz = rv.vz[i];
+    const double t = rv.t[i], w_ = rv.wavelength[i];
+    // the reference's phase at p0 (rayVector.cpp:207-214), z_p = 0
+    const double v2 = vx*vx + vy*vy + vz*vz;
+    double ph = (g.p0x-x)*vx;
+    ph += (g.p0y-y)*vy;
+    ph += (0.0-z)*vz;
+    ph /= v2;
+    ph -= g.t-t;
+    ph *= kTwoPi / w_;
+    double rev = ph / kTwoPi;
+    rev -= floor(rev);
+    // revolutions per pixel step along a1 and a2
+    const double inv = 1.0 / (v2 * w_);
+    const double al = (g.a1x*vx + g.a1y*vy) * inv;
+    const double be = (g.a2x*vx + g.a2y*vy) * inv;
+    const float ah = hi_bits((float)al, g.keep_a);
+    const float bh = hi_bits((float)be, g.keep_b);
+    float* o = params + (int64_t)rank * kRayStride;
+    o[0] = ah;
+    o[1] = (float)(al - (double)ah);
+    o[2] = bh;
+    o[3] = (float)(be - (double)bh);
+    o[4] = (float)rev;
+    o[5] = (float)rv.flux[i];
+    o[6] = 0.f;
+    o[7] = 0.f;
+}
+
+// sin(2 pi theta) with theta in revolutions (v_sin_f32).
+__device__ __forceinline__ float sin_rev(float th) { return __builtin_amdgcn_sinf(th); }
+__device__ __forceinline__ float cos_rev(float th) { return __builtin_amdgcn_cosf(th); }
+
+__device__ __forceinline__ float step_phase(float c, float hi, float lo, float add)
+{
+    // fract(c*hi) is exact: c*hi is exact by construction of hi
+    return __builtin_amdgcn_fractf(c * hi) + __builtin_fmaf(c, lo, add);
+}
+
+template <int kPrec>
+__global__ __launch_bounds__(256, 2) void k_psf_gemm(
+    const float* __restrict__ params, const int* __restrict__ total_ptr,
+    int nx, int ny, int tiles_i, int ksplit, int nxp, int nyp,
+    float* __restrict__ partial)
+{
+    const int tile = blockIdx.x;
+    const int ks = blockIdx.y;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int i0 = (tile % tiles_i) * kTile + (wave & 1) * 64;
+    const int j0 = (tile / tiles_i) * kTile + (wave >> 1) * 64;
+
+    const int total = *total_ptr;
+    int per = (total + ksplit - 1) / ksplit;
+    per = ((per + kRayGroup - 1) / kRayGroup) * kRayGroup;
+    const int r0 = ks * per;
+    const int r1 = min(total, r0 + per);
+
+    const int half = lane >> 5;       // k half of the operand
+    const int l32 = lane & 31;
+    v16f acc[4][2];
+    for (int a = 0; a < 4; a++)
+        for (int b = 0; b < 2; b++)
+            for (int e = 0; e < 16; e++) acc[a][b][e] = 0.f;
+
+    // per-lane pixel offsets of the A rows (i) and B columns (j)
+    float ci[4], cj[2];
+    const int prow = l32 & 1;                          // re / im row
+    for (int rb = 0; rb < 4; rb++) ci[rb] = (float)(i0 + rb*16 + (l32 >> 1) - nx/2);
+    for (int cb = 0; cb < 2; cb++) cj[cb] = (float)(j0 + cb*32 + l32 - ny/2);
+
+    if (kPrec == BT_PSF_FP32) {
+        // 32x32x2: lane supplies A'[l32][half], B'[half][l32]; one ray per MFMA
+        //   (prow, half) = (0,0): Re U = f cos;  (0,1): -Im U = -f sin
+        //                  (1,0): Im U = f sin;  (1,1): Re U = f cos
+        const float sa = (prow == 0 && half == 1) ? -1.f : 1.f;
+        const float oa = (prow == half) ? 0.25f : 0.f;
+        const float ob = (half == 0) ? 0.25f : 0.f;    // Re V = cos, Im V = sin
+        for (int r = r0; r < r1; r++) {
+            const float* p = params + (int64_t)r * kRayStride;
+            const float ah = p[0], al = p[1], bh = p[2], bl = p[3], ph0 = p[4], f = p[5];
+            float av[4], bv[2];
+            for (int rb = 0; rb < 4; rb++)
+                av[rb] = f * sin_rev(__builtin_fmaf(sa, step_phase(ci[rb], ah, al, ph0), oa));
+            for (int cb = 0; cb < 2; cb++)
+                bv[cb] = sin_rev(step_phase(cj[cb], bh, bl, ob));
+            for (int rb = 0; rb < 4; rb++)
+                for (int cb = 0; cb < 2; cb++)
+                    acc[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[rb], bv[cb],
+                                                                      acc[rb][cb], 0, 0, 0);
+        }
+    } else {
+        // 32x32x16 bf16: lane supplies A'[l32][8*half + e], e = 0..7, i.e. rays
+        // 4*half + e/2, part e%2; same for B'.
+        for (int r = r0; r < r1; r += kRayGroup) {
+            v8bf ahi[4], alo[4], bhi[2], blo[2];
+            for (int q = 0; q < 4; q++) {
+                const int rr = r + 4*half + q;
+                const bool live = rr < r1;
+                const float* p = params + (int64_t)(live ? rr : r0) * kRayStride;
+                const float ah = p[0], al = p[1], bh = p[2], bl = p[3], ph0 = p[4];
+                const float f = live ? p[5] : 0.f;
+                for (int rb = 0; rb < 4; rb++) {
+                    const float th = step_phase(ci[rb], ah, al, ph0);
+                    const float s = f * sin_rev(th), c = f * cos_rev(th);
+                    const float v0 = prow ? s : c;        // k part 0
+                    const float v1 = prow ? c : -s;       // k part 1
+                    const __bf16 h0 = (__bf16)v0, h1 = (__bf16)v1;
+                    ahi[rb][2*q] = h0;
+                    ahi[rb][2*q+1] = h1;
+                    alo[rb][2*q] = (__bf16)(v0 - (float)h0);
+                    alo[rb][2*q+1] = (__bf16)(v1 - (float)h1);
+                }
+                for (int cb = 0; cb < 2; cb++) {
+                    const float th = step_phase(cj[cb], bh, bl, 0.f);
+                    const float c = cos_rev(th), s = sin_rev(th);
+                    const __bf16 hc = (__bf16)c, hs = (__bf16)s;
+                    bhi[cb][2*q] = hc;
+                    bhi[cb][2*q+1] = hs;
+                    blo[cb][2*q] = (__bf16)(c - (float)hc);
+                    blo[cb][2*q+1] = (__bf16)(s - (float)hs);
+                }
+            }
+            for (int rb = 0; rb < 4; rb++)
+                for (int cb = 0; cb < 2; cb++) {
+                    v16f a = acc[rb][cb];
+                    a = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo[rb], bhi[cb], a, 0, 0, 0);
+                    a = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi[rb], blo[cb], a, 0, 0, 0);
+                    a = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi[rb], bhi[cb], a, 0, 0, 0);
+                    acc[rb][cb] = a;
+                }
+        }
+    }
+
+    // D[row][col]: row = 8*(e/4) + 4*half + e%4, col = l32 (32x32 MFMA layout)
+    // partial[ks][i][j][2] over the padded nxp x nyp image
+    float* out = partial + (int64_t)ks * nxp * nyp * 2;
+    for (int rb = 0; rb < 4; rb++)
+        for (int cb = 0; cb < 2; cb++)
+            for (int e = 0; e < 16; e++) {
+                const int row = 8*(e >> 2) + 4*half + (e & 3);
+                const int i = i0 + rb*16 + (row >> 1);
+                const int j = j0 + cb*32 + l32;
+                out[((int64_t)i * nyp + j) * 2 + (row & 1)] = acc[rb][cb][e];
+            }
+}
+
+// amp[j][i] = sum_ks partial[ks][i][j] in FP64, fixed order.
+__global__ __launch_bounds__(kBlock) void k_psf_reduce(const float* __restrict__ partial,
+                                                       int ksplit, int nx, int ny, int nxp,
+                                                       int nyp, double* __restrict__ amp)
+{
+    const int64_t pix = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (pix >= (int64_t)nx * ny) return;
+    const int j = (int)(pix / nx), i = (int)(pix % nx);
+    const int64_t stride = (int64_t)nxp * nyp * 2;
+    const float* p = partial + ((int64_t)i * nyp + j) * 2;
+    double re = 0.0, im = 0.0;
+    for (int k = 0; k < ksplit; k++) {
+        re += (double)p[k * stride];
+        im += (double)p[k * stride + 1];
+    }
+    amp[2*pix] = re;
+    amp[2*pix+1] = im;
+}
+
+struct GemmLayout {
+    int64_t nblocks;      // ray blocks of kBlock
+    int tiles_i, tiles_j, nxp, nyp, ksplit;
+    int64_t off_params, off_counts, off_offsets, off_partial, bytes;
+};
+
+int bits_for(int c)
+{
+    int b = 0;
+    while ((1 << b) <= c) b++;
+    return b;
+}
+
+GemmLayout gemm_layout(int64_t n, int nx, int ny)
+{
+    GemmLayout L;
+    L.nblocks = (n + kBlock - 1) / kBlock;
+    if (L.nblocks < 1) L.nblocks = 1;
+    L.tiles_i = (nx + kTile - 1) / kTile;
+    L.tiles_j = (ny + kTile - 1) / kTile;
+    L.nxp = L.tiles_i * kTile;
+    L.nyp = L.tiles_j * kTile;
+    const int tiles = L.tiles_i * L.tiles_j;
+    // about 512 workgroups (2048 waves: two per SIMD), at least 64 rays each
+    int ks = (512 + tiles - 1) / tiles;
+    int64_t by_rays = (n + 63) / 64;
+    if (by_rays < 1) by_rays = 1;
+    if (ks > by_rays) ks = (int)by_rays;
+    if (ks > 4096) ks = 4096;
+    if (ks < 1) ks = 1;
+    L.ksplit = ks;
+    auto align = [](int64_t v) { return (v + 255) & ~(int64_t)255; };
+    int64_t off = 0;
+    L.off_params = off;  off = align(off + (n + kRayGroup) * kRayStride * (int64_t)sizeof(float));
+    L.off_counts = off;  off = align(off + L.nblocks * (int64_t)sizeof(int));
+    L.off_offsets = off; off = align(off + (L.nblocks + 1) * (int64_t)sizeof(int));
+    L.off_partial = off; off = align(off + (int64_t)ks * L.nxp * L.nyp * 2 * (int64_t)sizeof(float));
+    L.bytes = off;
+    return L;
+}
+
+}  // namespace bt
+
+using namespace bt;
+
 extern "C" int64_t bt_huygens_workspace_bytes(int64_t n, int nx, int ny)
 {
-    return bt_huygens_direct_workspace_bytes(n, nx, ny);
+    if (n < 0 || nx <= 0 || ny <= 0) return 0;
+    const int64_t d = bt_huygens_direct_workspace_bytes(n, nx, ny);
+    const int64_t g = gemm_layout(n, nx, ny).bytes;
+    return d > g ? d : g;
 }
 
 extern "C" int bt_huygens_gemm(bt_rays rays, const double* p0, const double* a1,
@@ -15,8 +351,62 @@ extern "C" int bt_huygens_gemm(bt_rays rays, const double* p0, const double* a1,
                                int ignore_vignetted, int precision, double* amp_out,
                                void* workspace, int64_t workspace_bytes, void* stream)
 {
-    (void)rays; (void)p0; (void)a1; (void)a2; (void)nx; (void)ny; (void)t;
-    (void)ignore_vignetted; (void)precision; (void)amp_out; (void)workspace;
-    (void)workspace_bytes; (void)stream;
-    return bt::fail(BT_E_UNSUPPORTED, "bt_huygens_gemm: not built yet");
+    if (!rays_ok(rays) || !p0 || !a1 || !a2 || !amp_out || !workspace)
+        return fail(BT_E_NULL, "bt_huygens_gemm: null argument");
+    if (nx <= 0 || ny <= 0 || nx > 8192 || ny > 8192)
+        return fail(BT_E_SIZE, "bt_huygens_gemm: image size must be 1..8192 per side");
+    if (rays.n > (int64_t)1 << 30)
+        return fail(BT_E_SIZE, "bt_huygens_gemm: more than 2^30 rays");
+    if (precision != BT_PSF_FP32 && precision != BT_PSF_BF16X3)
+        return fail(BT_E_KIND, "bt_huygens_gemm: unknown precision");
+    const GemmLayout L = gemm_layout(rays.n, nx, ny);
+    if (workspace_bytes < L.bytes) return fail(BT_E_SIZE, "bt_huygens_gemm: workspace too small");
+    char* ws = static_cast<char*>(workspace);
+    float* params = reinterpret_cast<float*>(ws + L.off_params);
+    int* counts = reinterpret_cast<int*>(ws + L.off_counts);
+    int* offsets = reinterpret_cast<int*>(ws + L.off_offsets);
+    float* partial = reinterpret_cast<float*>(ws + L.off_partial);
+    hipStream_t s = as_stream(stream);
+
+    PsfGeom g;
+    g.p0x = p0[0]; g.p0y = p0[1];
+    g.a1x = a1[0]; g.a1y = a1[1];
+    g.a2x = a2[0]; g.a2y = a2[1];
+    g.t = t;
+    g.nx = nx; g.ny = ny;
+    // |ci| <= max(nx/2, nxp - nx/2): keep 24 - bits(|ci|) bits so ci*hi is exact
+    const int cmax_i = (L.nxp - nx / 2 > nx / 2) ? L.nxp - nx / 2 : nx / 2;
+    const int cmax_j = (L.nyp - ny / 2 > ny / 2) ? L.nyp - ny / 2 : ny / 2;
+    g.keep_a = 24 - bits_for(cmax_i);
+    g.keep_b = 24 - bits_for(cmax_j);
+
+    const unsigned nb = (unsigned)L.nblocks;
+    if (rays.n > 0) {
+        hipLaunchKernelGGL(k_psf_count, dim3(nb), dim3(kBlock), 0, s, rays, ignore_vignetted, counts);
+        int rc = check_launch("bt_huygens_gemm(count)");
+        if (rc) return rc;
+    }
+    hipLaunchKernelGGL(k_psf_scan, dim3(1), dim3(kScanThreads), 0, s, counts,
+                       rays.n > 0 ? (int)nb : 0, offsets);
+    int rc = check_launch("bt_huygens_gemm(scan)");
+    if (rc) return rc;
+    if (rays.n > 0) {
+        hipLaunchKernelGGL(k_psf_prep, dim3(nb), dim3(kBlock), 0, s, rays, g, ignore_vignetted,
+                           offsets, params);
+        rc = check_launch("bt_huygens_gemm(prep)");
+        if (rc) return rc;
+    }
+    const int* total = offsets + (rays.n > 0 ? nb : 0);
+    dim3 grid(L.tiles_i * L.tiles_j, L.ksplit);
+    if (precision == BT_PSF_FP32)
+        hipLaunchKernelGGL(k_psf_gemm<BT_PSF_FP32>, grid, dim3(256), 0, s, params, total, nx, ny,
+                           L.tiles_i, L.ksplit, L.nxp, L.nyp, partial);
+    else
+        hipLaunchKernelGGL(k_psf_gemm<BT_PSF_BF16X3>, grid, dim3(256), 0, s, params, total, nx, ny,
+                           L.tiles_i, L.ksplit, L.nxp, L.nyp, partial);
+    rc = check_launch("bt_huygens_gemm(mfma)");
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_psf_reduce, dim3(grid_for((int64_t)nx * ny)), dim3(kBlock), 0, s,
+                       partial, L.ksplit, nx, ny, L.nxp, L.nyp, amp_out);
+    return check_launch("bt_huygens_gemm(reduce)");
 }

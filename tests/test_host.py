@@ -1,0 +1,177 @@
+"""Host-side logic: Zernike -> monomial conversion, frames, YAML parsing, plan
+compilation, ray-grid rules.  CPU only."""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+import batoid_amd as bt
+import oracle
+from oracle.host import host_records
+from batoid_amd.plan import PlanSpec, collect_steps
+from batoid_amd.zernike import noll_to_nm, zernike_coef_arrays
+from batoid_amd import workloads
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def poly_eval(c, x, y):
+    """Brute-force sum c[i][k] x^i y^k."""
+    out = np.zeros_like(x)
+    for i in range(c.shape[0]):
+        for k in range(c.shape[1]):
+            out += c[i, k] * x**i * y**k
+    return out
+
+
+def zj(j, x, y, R_outer=1.0, R_inner=0.0):
+    coef = np.zeros(j + 1)
+    coef[j] = 1.0
+    c, gx, gy = zernike_coef_arrays(coef, R_outer, R_inner)
+    return poly_eval(c, x, y)
+
+
+def test_noll_indices():
+    expect = {1: (0, 0), 2: (1, 1), 3: (1, -1), 4: (2, 0), 5: (2, -2), 6: (2, 2),
+              7: (3, -1), 8: (3, 1), 9: (3, -3), 10: (3, 3), 11: (4, 0), 12: (4, 2),
+              13: (4, -2), 14: (4, 4), 15: (4, -4), 22: (6, 0), 28: (6, 6)}
+    for j, nm in expect.items():
+        assert noll_to_nm(j) == nm, j
+
+
+def test_zernike_closed_forms():
+    """Noll Z1..Z11 on the unit disk."""
+    rng = np.random.default_rng(1)
+    x, y = rng.uniform(-0.7, 0.7, (2, 200))
+    r2 = x * x + y * y
+    s3, s6, s8, s5 = np.sqrt(3), np.sqrt(6), np.sqrt(8), np.sqrt(5)
+    forms = {
+        1: np.ones_like(x), 2: 2 * x, 3: 2 * y, 4: s3 * (2 * r2 - 1),
+        5: s6 * 2 * x * y, 6: s6 * (x * x - y * y),
+        7: s8 * (3 * r2 - 2) * y, 8: s8 * (3 * r2 - 2) * x,
+        9: s8 * (3 * x * x * y - y**3), 10: s8 * (x**3 - 3 * x * y * y),
+        11: s5 * (6 * r2 * r2 - 6 * r2 + 1),
+    }
+    for j, f in forms.items():
+        np.testing.assert_allclose(zj(j, x, y), f, rtol=0, atol=1e-13, err_msg=f"Z{j}")
+
+
+def test_annular_defocus_closed_form():
+    """Mahajan's annular Z4 = sqrt(3) (2 rho^2 - 1 - e^2) / (1 - e^2)."""
+    eps = 0.61
+    rng = np.random.default_rng(2)
+    x, y = rng.uniform(-1, 1, (2, 300))
+    f = np.sqrt(3) * (2 * (x * x + y * y) - 1 - eps**2) / (1 - eps**2)
+    np.testing.assert_allclose(zj(4, x, y, 1.0, eps), f, rtol=0, atol=1e-12)
+
+
+@pytest.mark.parametrize("eps", [0.0, 0.35])
+def test_zernike_orthonormal_on_annulus(eps):
+    """<Z_i Z_j> over the annulus = delta_ij (Gauss-Legendre in r^2 x uniform
+    azimuth, exact for these polynomial degrees)."""
+    jmax = 28
+    nr, nt = 16, 40
+    u, w = np.polynomial.legendre.leggauss(nr)
+    r2 = eps**2 + (u + 1) / 2 * (1 - eps**2)
+    th = np.arange(nt) * 2 * np.pi / nt
+    R, T = np.meshgrid(np.sqrt(r2), th, indexing='ij')
+    W = np.repeat(w[:, None] / 2, nt, axis=1) / nt
+    x, y = R * np.cos(T), R * np.sin(T)
+    Z = np.array([zj(j, x, y, 1.0, eps) for j in range(1, jmax + 1)])
+    G = np.einsum('aij,bij,ij->ab', Z, Z, W)
+    np.testing.assert_allclose(G, np.eye(jmax), atol=1e-11)
+
+
+def test_zernike_gradients_and_scaling():
+    rng = np.random.default_rng(3)
+    coef = rng.normal(size=23)
+    R = 4.18
+    c, gx, gy = zernike_coef_arrays(coef, R, 0.3 * R)
+    x, y = rng.uniform(-3, 3, (2, 100))
+    h = 1e-6
+    dx = (poly_eval(c, x + h, y) - poly_eval(c, x - h, y)) / (2 * h)
+    dy = (poly_eval(c, x, y + h) - poly_eval(c, x, y - h)) / (2 * h)
+    np.testing.assert_allclose(poly_eval(gx, x, y), dx, rtol=1e-6, atol=1e-8)
+    np.testing.assert_allclose(poly_eval(gy, x, y), dy, rtol=1e-6, atol=1e-8)
+    # scaling: the series at (x, y) equals the unit-radius series at (x/R, y/R)
+    c1, _, _ = zernike_coef_arrays(coef, 1.0, 0.3)
+    np.testing.assert_allclose(poly_eval(c, x, y), poly_eval(c1, x / R, y / R), rtol=1e-12)
+
+
+def test_polynomial_surface_evaluation_matches_monomials():
+    """The oracle's 2-D Horner (polynomialSurface.cpp:50-66) equals the
+    brute-force monomial sum, for sag and normal."""
+    z = bt.Zernike(np.random.default_rng(4).normal(scale=1e-3, size=29), R_outer=1.3,
+                   x_origin=0.01, y_origin=-0.02)
+    rng = np.random.default_rng(5)
+    x, y = rng.uniform(-1, 1, (2, 500))
+    tab = host_records(z)
+    out = np.empty(500)
+    oracle.load().or_sag_n(tab.ptr(), x.ctypes.data, y.ctypes.data, out.ctypes.data, 500, 1)
+    np.testing.assert_allclose(out, poly_eval(z._xycoef, x - 0.01, y + 0.02), rtol=1e-12, atol=1e-18)
+
+
+def test_coordsys_and_transform():
+    cs = bt.CoordSys().shiftLocal([0.1, -0.2, 0.3]).rotateLocal(bt.RotX(0.1) @ bt.RotZ(0.3))
+    ct = bt.CoordTransform(bt.CoordSys(), cs)
+    rng = np.random.default_rng(6)
+    x, y, z = rng.normal(size=(3, 10))
+    fwd = ct.applyForwardArray(x, y, z)
+    np.testing.assert_allclose(fwd.T, cs.toLocal(np.array([x, y, z]).T), atol=1e-14)
+    back = ct.applyReverseArray(*fwd)
+    np.testing.assert_allclose(back, [x, y, z], atol=1e-14)
+    np.testing.assert_allclose(cs.rot @ cs.rot.T, np.eye(3), atol=1e-15)
+
+
+def test_yaml_and_plan():
+    optic = bt.Optic.fromYaml(os.path.join(GOLD, "optics", "LSST_r.yaml"))
+    steps = collect_steps(optic)
+    assert [s[0].name for s in steps] == [
+        'M1', 'M2', 'M3', 'L1_entrance', 'L1_exit', 'L2_entrance', 'L2_exit',
+        'Filter_entrance', 'Filter_exit', 'L3_entrance', 'L3_exit', 'Detector']
+    assert [s[1] for s in steps] == [1] * 3 + [2] * 8 + [0]
+    spec = PlanSpec(steps, bt.CoordSys())
+    assert len(spec.steps) == 12 and spec.flags == 0
+    assert len(spec.med_recs) == 2                     # silica and vacuum, shared
+    assert isinstance(optic['L2_exit'].surface, bt.Asphere)
+    assert isinstance(optic['M1'].obscuration, bt.ObscNegation)
+    # reverse traces the items backwards and swaps media
+    rsteps = collect_steps(optic, reverse=True)
+    assert rsteps[0][0].name == 'Detector' and rsteps[-1][0].name == 'M1'
+    assert rsteps[1][2] == optic['L3_exit'].inMedium
+    # skipping removes the item from the plan
+    optic['L2'].skip = True
+    assert len(collect_steps(optic)) == 10
+    optic['L2'].skip = False
+
+
+def test_tma_workload_matches_lsst_prescription():
+    tma = workloads.tma(m1_coefs=(0.0, -1.381e-9))
+    lsst = bt.Optic.fromYaml(os.path.join(GOLD, "optics", "LSST_r.yaml"))
+    a = PlanSpec(collect_steps(tma), bt.CoordSys())
+    b = PlanSpec(collect_steps(lsst), bt.CoordSys())
+    for sa, sb in zip(a.steps, b.steps):
+        assert bytes(sa) == bytes(sb)
+    assert [bytes(r) for r in a.surf_recs] == [bytes(r) for r in b.surf_recs]
+
+
+def test_hexapolar_pupil_size():
+    rr, th = bt.utils.hexapolar(outer=4.18, inner=2.558, nrad=1120, naz=11080, rth=True)
+    assert len(rr) == 9998544
+    assert rr.max() == 4.18 and rr.min() == 2.558
+
+
+def test_lattice_convention():
+    lat = bt.Lattice(np.zeros((4, 5)), np.array([[1.0, 0.0], [0.0, 2.0]]))
+    np.testing.assert_array_equal(lat.coords[0, 0], [-2, -4])
+    np.testing.assert_array_equal(lat.coords[3, 4], [1, 4])
+
+
+def test_field_projections_round_trip():
+    u, v = 0.01, -0.02
+    for proj in ('postel', 'zemax', 'gnomonic', 'stereographic', 'lambert', 'orthographic'):
+        a, b, c = bt.utils.fieldToDirCos(u, v, projection=proj)
+        np.testing.assert_allclose(a * a + b * b + c * c, 1, rtol=1e-14)
+        np.testing.assert_allclose(bt.utils.dirCosToField(a, b, c, projection=proj), (u, v),
+                                   rtol=1e-12)
