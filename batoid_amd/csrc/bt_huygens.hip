@@ -205,18 +205,37 @@ __global__ __launch_bounds__(256, 2) void k_psf_gemm(
         const float sa = (prow == 0 && half == 1) ? -1.f : 1.f;
         const float oa = (prow == half) ? 0.25f : 0.f;
         const float ob = (half == 0) ? 0.25f : 0.f;    // Re V = cos, Im V = sin
-        for (int r = r0; r < r1; r++) {
-            const float* p = params + (int64_t)r * kRayStride;
-            const float ah = p[0], al = p[1], bh = p[2], bl = p[3], ph0 = p[4], f = p[5];
-            float av[4], bv[2];
-            for (int rb = 0; rb < 4; rb++)
-                av[rb] = f * sin_rev(__builtin_fmaf(sa, step_phase(ci[rb], ah, al, ph0), oa));
-            for (int cb = 0; cb < 2; cb++)
-                bv[cb] = sin_rev(step_phase(cj[cb], bh, bl, ob));
-            for (int rb = 0; rb < 4; rb++)
+        // rays in groups of 4 whose parameters (wave-uniform) are fetched one
+        // group ahead of their use
+        constexpr int G = 4;
+        float cur[G][6], nxt[G][6];
+        auto fetch = [&](float (&dst)[G][6], int rbase) {
+            for (int q = 0; q < G; q++) {
+                const int rr = rbase + q;
+                const bool live = rr < r1;
+                const float* p = params + (int64_t)(live ? rr : r0) * kRayStride;
+                for (int e = 0; e < 5; e++) dst[q][e] = p[e];
+                dst[q][5] = live ? p[5] : 0.f;
+            }
+        };
+        if (r0 < r1) fetch(cur, r0);
+        for (int r = r0; r < r1; r += G) {
+            if (r + G < r1) fetch(nxt, r + G);
+            for (int q = 0; q < G; q++) {
+                const float ah = cur[q][0], al = cur[q][1], bh = cur[q][2], bl = cur[q][3];
+                const float ph0 = cur[q][4], f = cur[q][5];
+                float av[4], bv[2];
+                for (int rb = 0; rb < 4; rb++)
+                    av[rb] = f * sin_rev(__builtin_fmaf(sa, step_phase(ci[rb], ah, al, ph0), oa));
                 for (int cb = 0; cb < 2; cb++)
-                    acc[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[rb], bv[cb],
-                                                                      acc[rb][cb], 0, 0, 0);
+                    bv[cb] = sin_rev(step_phase(cj[cb], bh, bl, ob));
+                for (int rb = 0; rb < 4; rb++)
+                    for (int cb = 0; cb < 2; cb++)
+                        acc[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[rb], bv[cb],
+                                                                          acc[rb][cb], 0, 0, 0);
+            }
+            for (int q = 0; q < G; q++)
+                for (int e = 0; e < 6; e++) cur[q][e] = nxt[q][e];
         }
     } else {
         // 32x32x16 bf16: lane supplies A'[l32][8*half + e], e = 0..7, i.e. rays
@@ -274,21 +293,24 @@ __global__ __launch_bounds__(256, 2) void k_psf_gemm(
             }
 }
 
-// amp[j][i] = sum_ks partial[ks][i][j] in FP64, fixed order.
+// amp[j][i] = sum_ks partial[ks][i][j] in FP64, fixed order.  Threads walk
+// j fastest so the partial-image reads are coalesced float2 loads.
 __global__ __launch_bounds__(kBlock) void k_psf_reduce(const float* __restrict__ partial,
                                                        int ksplit, int nx, int ny, int nxp,
                                                        int nyp, double* __restrict__ amp)
 {
-    const int64_t pix = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (pix >= (int64_t)nx * ny) return;
-    const int j = (int)(pix / nx), i = (int)(pix % nx);
-    const int64_t stride = (int64_t)nxp * nyp * 2;
-    const float* p = partial + ((int64_t)i * nyp + j) * 2;
+    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (t >= (int64_t)nx * ny) return;
+    const int i = (int)(t / ny), j = (int)(t % ny);
+    const int64_t stride = (int64_t)nxp * nyp;
+    const float2* p = reinterpret_cast<const float2*>(partial) + ((int64_t)i * nyp + j);
     double re = 0.0, im = 0.0;
     for (int k = 0; k < ksplit; k++) {
-        re += (double)p[k * stride];
-        im += (double)p[k * stride + 1];
+        const float2 v = p[k * stride];
+        re += (double)v.x;
+        im += (double)v.y;
     }
+    const int64_t pix = (int64_t)j * nx + i;
     amp[2*pix] = re;
     amp[2*pix+1] = im;
 }

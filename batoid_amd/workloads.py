@@ -111,3 +111,27 @@ def newtonian_rays(optic, n=1_000_000, seed=57721, wavelength=500e-9, theta_deg=
     return RayVector.asPolar(optic=optic, outer=1.0, inner=0.0, nrandom=n,
                              rng=np.random.default_rng(seed), wavelength=wavelength,
                              theta_x=np.deg2rad(theta_deg), theta_y=0.0)
+
+
+def perturbed_tma(jmax=28, zernike_rms=50e-9, zernike_seed=577215, grid=204,
+                  grid_rms=10e-9, grid_seed=5772156649, smoothing=4.0, **kwargs):
+    """The TMA with every mirror a Sum of its Asphere, a Zernike series
+    (j <= 28, coefficients N(0, 50 nm), R_outer = the mirror's clear outer
+    radius) and a Bicubic 204 x 204 grid over +-1.02 R_outer of Gaussian-
+    smoothed white noise scaled to 10 nm rms."""
+    from scipy.ndimage import gaussian_filter
+    from .surface import Sum, Zernike, Bicubic
+    optic = tma(**kwargs)
+    zrng = np.random.default_rng(zernike_seed)
+    brng = np.random.default_rng(grid_seed)
+    for name in ("M1", "M2", "M3"):
+        iface = optic[name]
+        R = iface.R_outer
+        coef = zrng.normal(0.0, zernike_rms, size=jmax + 1)
+        coef[0] = 0.0
+        xs = np.linspace(-1.02 * R, 1.02 * R, grid)
+        noise = gaussian_filter(brng.normal(size=(grid, grid)), smoothing)
+        noise *= grid_rms / np.std(noise)
+        surf = Sum(iface.surface, Zernike(coef, R_outer=R), Bicubic(xs, xs, noise))
+        optic = optic.withSurface(name, surf)
+    return optic

@@ -25,6 +25,8 @@ import numpy as np
 METRIC = "ray-surface intersections/s (FP64, ≤1e-12 m) + Huygens PSF TFLOP/s vs plain"
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+FP32_MFMA_PEAK_TFLOPS = 157.3
+BF16_MFMA_PEAK_TFLOPS = 2500.0
 # Bytes the fused trace moves per ray: reads x, y, z, vx, vy, vz, t,
 # wavelength (8 B each) + 2 flag bytes; writes x..t + 2 flag bytes.
 BYTES_PER_RAY = 8 * 8 + 2 + 7 * 8 + 2
@@ -85,6 +87,70 @@ def cpu_baseline(optic, rv, target_s=10.0):
                       f"with {threads} OpenMP threads"}
 
 
+def bench_huygens(optic, reps=10):
+    """C4: the TMA pupil on a 512 x 512 grid (one field, 620 nm), traced, then
+    a 256 x 256 Huygens PSF at 0.5 um pixels by the MFMA GEMM (FP32 and
+    split-BF16) and by the FP64 direct sum.  TFLOP/s counts 8 flops per
+    complex multiply-add: 8 * pixels * (unvignetted rays)."""
+    import torch
+    import batoid_amd as bt
+    from batoid_amd import _lib, _device, workloads
+    thx, thy = workloads.tma_fields()
+    dirCos = bt.utils.fieldToDirCos(thx[0], thy[0])
+    rays = bt.RayVector.asGrid(optic=optic, wavelength=620e-9, dirCos=dirCos, nx=512)
+    optic.trace(rays)
+    nvalid = int((~rays.vignetted & ~rays.failed).sum())
+    cr = (512 // 2) * 512 + 512 // 2
+    point = rays.r[cr]
+    time_ = float(rays.t[0])
+    npix = 256
+    prim = np.array([[0.5e-6, 0.0], [0.0, 0.5e-6]])
+    lib = _lib.load()
+    ws_bytes = int(lib.bt_huygens_workspace_bytes(len(rays), npix, npix))
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=_device.device())
+    out = torch.empty(2 * npix * npix, dtype=torch.float64, device=_device.device())
+    args = (rays._rays, _lib.dvec(point[:2]), _lib.dvec(prim[0]), _lib.dvec(prim[1]),
+            npix, npix, time_, 1)
+    stream = _device.stream()
+
+    def run(kind):
+        if kind == "direct":
+            return lib.bt_huygens_direct(*args, _device.ptr(out), _device.ptr(ws), ws_bytes, stream)
+        prec = _lib.PSF_FP32 if kind == "fp32" else _lib.PSF_BF16X3
+        return lib.bt_huygens_gemm(*args, prec, _device.ptr(out), _device.ptr(ws), ws_bytes,
+                                   stream)
+
+    res = {"rays_traced": len(rays), "rays_summed": nvalid, "pixels": npix * npix,
+           "pixel_um": 0.5, "wavelength_nm": 620,
+           "flop_per_psf": 8.0 * npix * npix * nvalid}
+    images = {}
+    for kind, n_rep in (("direct", 1), ("fp32", reps), ("bf16x3", reps)):
+        _lib.check(run(kind), kind)
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(n_rep):
+            run(kind)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / n_rep
+        amp = out.cpu().numpy().view(np.complex128).reshape(npix, npix)
+        images[kind] = np.abs(amp) ** 2
+        res[kind] = {"ms_per_psf": ms, "tflops": res["flop_per_psf"] / (ms * 1e-3) / 1e12}
+        log(f"huygens {kind}: {ms:.3f} ms/PSF, {res[kind]['tflops']:.2f} TFLOP/s")
+    ref = images["direct"]
+    for kind in ("fp32", "bf16x3"):
+        res[kind]["rel_l2_vs_fp64"] = float(np.sqrt(np.sum((images[kind] - ref) ** 2) /
+                                                     np.sum(ref ** 2)))
+    res["fp32"]["frac_of_fp32_mfma_peak"] = res["fp32"]["tflops"] / FP32_MFMA_PEAK_TFLOPS
+    # split-BF16 issues three bf16 products per complex MAC's real products
+    res["bf16x3"]["frac_of_bf16_mfma_peak"] = 3 * res["bf16x3"]["tflops"] / BF16_MFMA_PEAK_TFLOPS
+    res["speedup_fp32_vs_fp64_direct"] = res["direct"]["ms_per_psf"] / res["fp32"]["ms_per_psf"]
+    res["speedup_bf16x3_vs_fp64_direct"] = res["direct"]["ms_per_psf"] / res["bf16x3"]["ms_per_psf"]
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -94,6 +160,10 @@ def main():
     ap.add_argument("--nrad", type=int, default=1120)
     ap.add_argument("--naz", type=int, default=11080)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-huygens", action="store_true")
+    ap.add_argument("--calib-copy-mb", type=int, default=0,
+                    help="also run a plain device copy of this many MiB (calibrates "
+                         "FETCH_SIZE / WRITE_SIZE counters against known bytes)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -145,6 +215,14 @@ def main():
         step(i)
     torch.cuda.synchronize()
     log(f"warmup done ({args.warmup} steps)")
+    if args.calib_copy_mb:
+        nb = args.calib_copy_mb << 20
+        src = torch.ones(nb // 8, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"calibration copy: {nb} bytes read, {nb} bytes written")
+        del src, dst
 
     starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
     ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
@@ -202,6 +280,8 @@ def main():
             "kernel_ms": kernel_ms,
         },
     }
+    if not args.no_huygens:
+        result["huygens"] = bench_huygens(optic)
     if not args.no_cpu:
         log("timing the CPU baseline ...")
         result["cpu_baseline"] = cpu_baseline(optic, batches[0])
