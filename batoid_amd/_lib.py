@@ -62,6 +62,7 @@ class BtStep(ctypes.Structure):
 class BtPlan(ctypes.Structure):
     _fields_ = [
         ("nstep", ctypes.c_int32), ("flags", ctypes.c_int32),
+        ("nmedia", ctypes.c_int32), ("pad_", ctypes.c_int32),
         ("steps", ctypes.c_void_p), ("surfaces", ctypes.c_void_p),
         ("media", ctypes.c_void_p), ("obscs", ctypes.c_void_p),
     ]

@@ -315,7 +315,19 @@ BT_DEV void surf_normal(const bt_surface* s, double x, double y,
     ny = sy*nz;
 }
 
+BT_DEV bool same_bits(double a, double b)
+{
+    return __double_as_longlong(a) == __double_as_longlong(b);
+}
+
 // Tangent-plane Newton with exactly `niter` iterations (surface.cpp:22-49).
+//
+// Each iteration is a pure function of dt (the ray origin and direction are
+// fixed): dt -> rP(dt), sag(rP), normal(rP) -> dt'.  So once an iterate
+// repeats bit for bit -- a fixed point dt' == dt, or a two-cycle between
+// neighbouring doubles dt'' == dt -- every remaining iterate is known, and the
+// loop stops with exactly the state (dt, rP, sag) that running all `niter`
+// iterations would leave.  Results are bit-identical to the full loop.
 BT_DEV bool newton(const bt_surface* s, double x, double y, double z,
                    double vx, double vy, double vz, double& dt, int niter)
 {
@@ -323,15 +335,27 @@ BT_DEV bool newton(const bt_surface* s, double x, double y, double z,
     double rPy = y+vy*dt;
     double rPz = z+vz*dt;
     double sz = surf_sag(s, rPx, rPy);
+    double pdt = 0.0;            // the iterate before the current one (iter >= 1)
     for (int iter = 0; iter < niter; iter++) {
         double nx, ny, nz;
         surf_normal(s, rPx, rPy, nx, ny, nz);
-        dt = (rPx-x)*nx + (rPy-y)*ny + (sz-z)*nz;
-        dt /= (nx*vx + ny*vy + nz*vz);
+        double dtn = (rPx-x)*nx + (rPy-y)*ny + (sz-z)*nz;
+        dtn /= (nx*vx + ny*vy + nz*vz);
+        if (same_bits(dtn, dt)) break;            // fixed point: state unchanged
+        const double odt = dt, oz = rPz, osz = sz;
+        dt = dtn;
         rPx = x+vx*dt;
         rPy = y+vy*dt;
         rPz = z+vz*dt;
         sz = surf_sag(s, rPx, rPy);
+        if (iter >= 1 && same_bits(dt, pdt)) {
+            // two-cycle: iterates alternate between this state and the
+            // previous one; after the remaining (niter-1-iter) iterations the
+            // loop ends on this state if that count is even, else on the other
+            if ((niter - 1 - iter) & 1) { dt = odt; rPz = oz; sz = osz; }
+            break;
+        }
+        pdt = odt;
     }
     return fabs(sz-rPz) < 1e-12;
 }
@@ -584,10 +608,14 @@ struct Ray {
     bool vig, fail;
 };
 
-// Returns true when the ray's kinematic state was (re)written.
+// Returns true when the ray's kinematic state was (re)written.  `n2` is the
+// outgoing medium's index at the ray's wavelength (used by REFRACT only); the
+// reference evaluates it after the intersection (batoid.cpp:474), but it is
+// a pure function of the wavelength, so callers may evaluate it once per ray
+// and medium and reuse it across Interfaces.
 BT_DEV bool apply_interaction(int action, const double* __restrict__ dr,
                               const double* __restrict__ drot,
-                              const bt_surface* surf, const bt_medium* m2,
+                              const bt_surface* surf, double n2,
                               Ray& r, int niter)
 {
     if (r.fail) return false;
@@ -637,7 +665,6 @@ BT_DEV bool apply_interaction(int action, const double* __restrict__ dr,
             nx *= -1; ny *= -1; nz *= -1;
             alpha *= -1;
         }
-        double n2 = medium_n(m2, r.w);
         double eta = n1/n2;
         double sinsqr = eta*eta*(1-alpha*alpha);
         if (sinsqr > 1.0) {                          // total internal reflection

@@ -146,8 +146,12 @@ __global__ __launch_bounds__(kBlock) void k_step(
     r.x = rv.x[i]; r.y = rv.y[i]; r.z = rv.z[i];
     r.vx = rv.vx[i]; r.vy = rv.vy[i]; r.vz = rv.vz[i];
     r.t = rv.t[i];
-    r.w = (action == BT_ACT_REFRACT && !r.fail) ? rv.wavelength[i] : 0.0;
-    const bool moved = apply_interaction(action, xf.dr, xf.drot, surf, m2, r, niter);
+    double n2 = 0.0;
+    if (action == BT_ACT_REFRACT && !r.fail) {
+        r.w = rv.wavelength[i];
+        n2 = medium_n(m2, r.w);
+    }
+    const bool moved = apply_interaction(action, xf.dr, xf.drot, surf, n2, r, niter);
     if (moved) {
         rv.x[i] = r.x; rv.y[i] = r.y; rv.z[i] = r.z;
         rv.vx[i] = r.vx; rv.vy[i] = r.vy; rv.vz[i] = r.vz;
@@ -168,7 +172,23 @@ __global__ __launch_bounds__(kBlock) void k_obscure(
     if (obsc_contains(obsc, obsc_len, rv.x[i], rv.y[i])) rv.vignetted[i] = 1;
 }
 
-// Every step of a plan applied while the ray stays in registers.
+// Every step of a plan applied while the ray stays in registers.  A ray's
+// wavelength never changes, so the index of each of the plan's media is
+// evaluated once per ray up front (at most kPlanMedia media; larger plans
+// evaluate the medium at each refraction).
+constexpr int kPlanMedia = 4;
+
+__device__ __forceinline__ double pick_medium(const double (&nm)[kPlanMedia], int m)
+{
+    // m is wave-uniform: a scalar branch, no dynamic register indexing
+    switch (m) {
+    case 0: return nm[0];
+    case 1: return nm[1];
+    case 2: return nm[2];
+    default: return nm[3];
+    }
+}
+
 __global__ __launch_bounds__(kBlock) void k_plan(
     bt_plan plan, bt_rays in, bt_rays out, int niter, int need_w)
 {
@@ -181,14 +201,31 @@ __global__ __launch_bounds__(kBlock) void k_plan(
     r.vx = in.vx[i]; r.vy = in.vy[i]; r.vz = in.vz[i];
     r.t = in.t[i];
     r.w = need_w ? in.wavelength[i] : 0.0;
+    const int nmed = plan.nmedia;
+    const bool cached = nmed <= kPlanMedia;
+    double nm[kPlanMedia] = {0.0, 0.0, 0.0, 0.0};
+    if (need_w && cached && !r.fail) {
+        for (int m = 0; m < nmed; m++) {
+            const double v = medium_n(plan.media + m, r.w);
+            switch (m) {
+            case 0: nm[0] = v; break;
+            case 1: nm[1] = v; break;
+            case 2: nm[2] = v; break;
+            default: nm[3] = v; break;
+            }
+        }
+    }
     const int nstep = plan.nstep;
     for (int k = 0; k < nstep; k++) {
         if (r.fail && r.vig) break;   // nothing below can change this ray
         const bt_step* __restrict__ st = plan.steps + k;
         const int action = st->action;
-        const int mi = st->medium;
-        apply_interaction(action, st->dr, st->drot, plan.surfaces + st->surface,
-                          mi >= 0 ? plan.media + mi : nullptr, r, niter);
+        double n2 = 0.0;
+        if (action == BT_ACT_REFRACT && !r.fail) {
+            const int mi = st->medium;
+            n2 = cached ? pick_medium(nm, mi) : medium_n(plan.media + mi, r.w);
+        }
+        apply_interaction(action, st->dr, st->drot, plan.surfaces + st->surface, n2, r, niter);
         const int olen = st->obsc_len;
         if (olen > 0 && !r.vig)
             r.vig = obsc_contains(plan.obscs + st->obsc, olen, r.x, r.y);
@@ -402,6 +439,8 @@ int bt_trace_plan(bt_plan plan, bt_rays in, bt_rays out, int niter, void* stream
     if (plan.nstep < 0) return fail(BT_E_PLAN, "bt_trace_plan: negative step count");
     if (plan.nstep > 0 && (!plan.steps || !plan.surfaces))
         return fail(BT_E_PLAN, "bt_trace_plan: plan tables missing");
+    if (plan.nmedia < 0 || (plan.nmedia > 0 && !plan.media))
+        return fail(BT_E_PLAN, "bt_trace_plan: bad media table");
     if (!rays_ok(in) || !rays_ok(out)) return fail(BT_E_NULL, "bt_trace_plan: incomplete ray columns");
     if (in.n != out.n) return fail(BT_E_SIZE, "bt_trace_plan: in/out sizes differ");
     if (niter < 0) return fail(BT_E_SIZE, "bt_trace_plan: negative niter");

@@ -95,6 +95,7 @@ class DevicePlan:
         p = _lib.BtPlan()
         p.nstep = len(spec.steps)
         p.flags = spec.flags
+        p.nmedia = len(spec.med_recs)
         p.steps = self.steps.data_ptr() if spec.steps else None
         p.surfaces = self.surfaces.buffer.data_ptr() if self.surfaces else None
         p.media = self.media.buffer.data_ptr() if self.media else None

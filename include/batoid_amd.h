@@ -168,6 +168,8 @@ typedef struct bt_step {
 typedef struct bt_plan {
     int32_t nstep;
     int32_t flags;
+    int32_t nmedia;               /* records in the medium table             */
+    int32_t pad_;
     const bt_step* steps;         /* device */
     const bt_surface* surfaces;   /* device */
     const bt_medium* media;       /* device */

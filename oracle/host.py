@@ -174,6 +174,7 @@ class HostPlan:
         p = L.BtPlan()
         p.nstep = len(spec.steps)
         p.flags = spec.flags
+        p.nmedia = len(spec.med_recs)
         p.steps = ctypes.addressof(self.steps)
         p.surfaces = self.surfaces.ptr()
         p.media = self.media.ptr() if self.media else None

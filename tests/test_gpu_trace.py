@@ -221,3 +221,19 @@ def test_fused_plan_matches_per_surface_and_oracle():
     for item in [m1, l1.items[0], l1.items[1], baffle, det]:
         item.trace(rv2)
     assert_cols_identical(_host(rv2), ref.cols, what="per-surface")
+
+
+def test_perturbed_tma_bit_exact():
+    """C3 optic: every mirror a Sum(Asphere, Zernike j<=28, Bicubic 204^2);
+    the fused trace equals the oracle bit for bit."""
+    from batoid_amd import workloads
+    optic = workloads.perturbed_tma()
+    thx, thy = workloads.tma_fields()
+    for k, wl in ((0, 350e-9), (7, 1050e-9)):
+        rv = workloads.tma_rays(optic, thx[k], thy[k], wl, nrad=60, naz=600)
+        ref = HostRays.from_rv(rv)
+        HostPlan(PlanSpec(collect_steps(optic), rv.coordSys)).trace(ref, niter=NITER)
+        optic.trace(rv)
+        got = _host(rv)
+        assert_cols_identical(got, ref.cols, what=f"perturbed TMA field {k}")
+        assert 0 < got["vignetted"].sum() < len(rv)
