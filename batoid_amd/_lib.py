@@ -54,8 +54,9 @@ class BtStep(ctypes.Structure):
     _fields_ = [
         ("action", ctypes.c_int32), ("surface", ctypes.c_int32),
         ("medium", ctypes.c_int32), ("obsc", ctypes.c_int32),
-        ("obsc_len", ctypes.c_int32), ("pad_", ctypes.c_int32),
+        ("obsc_len", ctypes.c_int32), ("start", ctypes.c_int32),
         ("dr", ctypes.c_double * 3), ("drot", ctypes.c_double * 9),
+        ("mlp", ctypes.c_void_p),
     ]
 
 
@@ -76,7 +77,9 @@ MED_CONST, MED_TABLE, MED_SELLMEIER, MED_SUMITA, MED_AIR = range(5)
 (OB_CIRCLE, OB_ANNULUS, OB_RECTANGLE, OB_RAY, OB_POLYGON, OB_NEGATION,
  OB_UNION, OB_INTERSECTION) = range(8)
 ACT_INTERSECT, ACT_REFLECT, ACT_REFRACT = range(3)
+START_REFERENCE, START_CONIC, START_MLP = range(3)
 PLAN_NO_REFRACT = 1
+PLAN_FAST_START = 2
 PSF_FP32, PSF_BF16X3 = 0, 1
 OBSC_MAX_STACK = 32
 MAX_SUM_CHILDREN = 8
@@ -111,6 +114,16 @@ SIGNATURES = {
     "bt_amplitude": (_I, [BtRays, _D, _D, _D, _D, _V, _V]),
     "bt_sum_amplitude_workspace_bytes": (_L, [_L]),
     "bt_sum_amplitude": (_I, [BtRays, _D, _D, _D, _D, _I, _V, _V, _V]),
+    "bt_mlp_param_count": (_I, []),
+    "bt_mlp_packed_bytes": (_L, []),
+    "bt_mlp_grad_workspace_bytes": (_L, [_L]),
+    "bt_mlp_loss_grad": (_I, [_V, _V, _V, _V, _L, _V, _V, _V, _V, _L, _V]),
+    "bt_mlp_adam": (_I, [_V, _V, _V, _V, _L, ctypes.c_float, ctypes.c_float,
+                         ctypes.c_float, ctypes.c_float, _I, _V]),
+    "bt_mlp_forward": (_I, [_V, _V, _V, _L, _V, _V]),
+    "bt_mlp_pack": (_I, [_V, _V, _V, _V]),
+    "bt_mlp_forward_packed": (_I, [_V, _V, _L, _V, _V]),
+    "bt_newton_samples": (_I, [_V, _V, _V, BtRays, _I, _V, _V, _V, _V, _V]),
     "bt_huygens_workspace_bytes": (_L, [_L, _I, _I]),
     "bt_huygens_direct": (_I, [BtRays, _V, _V, _V, _I, _I, _D, _I, _V, _V, _L, _V]),
     "bt_huygens_gemm": (_I, [BtRays, _V, _V, _V, _I, _I, _D, _I, _I, _V, _V, _L, _V]),

@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include "../../include/batoid_amd.h"
+#include "bt_mlp.h"
 
 #define BT_DEV __device__ __forceinline__
 
@@ -466,6 +467,78 @@ BT_DEV bool surf_tti(const bt_surface* s, double x, double y, double z,
     return newton(s, x, y, z, vx, vy, vz, dt, niter);
 }
 
+// The surface's analytic base start: the root of a Sum's first child or of an
+// Asphere's conic (the starts of sum.cpp:53 and asphere.cpp:82, without the
+// Newton refinement that follows them); surfaces without one start at 0.
+BT_DEV bool conic_start(const bt_surface* s, double x, double y, double z, double vx,
+                        double vy, double vz, double& dt)
+{
+    const bt_surface* c = (s->kind == BT_SUM) ? s + 1 : s;
+    dt = 0.0;
+    switch (c->kind) {
+    case BT_ASPHERE:
+    case BT_QUADRIC:
+        return quadric_tti(c, x, y, z, vx, vy, vz, dt);
+    case BT_PLANE:
+    case BT_SPHERE:
+    case BT_PARABOLOID:
+    case BT_TILTED:
+        return leaf_tti(c, x, y, z, vx, vy, vz, dt, 0);
+    default:
+        return true;
+    }
+}
+
+
+BT_DEV bool newton_surface(const bt_surface* s)
+{
+    return s->kind == BT_SUM || s->kind == BT_ASPHERE || s->kind == BT_BICUBIC ||
+           s->kind == BT_POLYNOMIAL;
+}
+
+constexpr double kFastTol = 1e-13;   // sag gap accepted by the fast starts [m]
+constexpr int kFastIter = 8;
+
+// Newton on the full surface from `dt` until |sag - z| <= kFastTol.
+BT_DEV bool newton_fast(const bt_surface* s, double x, double y, double z,
+                        double vx, double vy, double vz, double& dt)
+{
+    double rPx = x+vx*dt, rPy = y+vy*dt, rPz = z+vz*dt;
+    double sz = surf_sag(s, rPx, rPy);
+    for (int it = 0; it < kFastIter; it++) {
+        if (fabs(sz-rPz) <= kFastTol) return true;
+        double nx, ny, nz;
+        surf_normal(s, rPx, rPy, nx, ny, nz);
+        double d = (rPx-x)*nx + (rPy-y)*ny + (sz-z)*nz;
+        dt = d / (nx*vx + ny*vy + nz*vz);
+        rPx = x+vx*dt;
+        rPy = y+vy*dt;
+        rPz = z+vz*dt;
+        sz = surf_sag(s, rPx, rPy);
+    }
+    return fabs(sz-rPz) <= kFastTol;
+}
+
+// Intersection with a fast start.  BT_START_CONIC seeds Newton with the base
+// conic's root; BT_START_MLP with `seed` (the caller's conic root plus the
+// network's correction, NaN when the conic has no root).  The reference path
+// decides every ray the fast start does not settle.
+BT_DEV bool fast_tti(const bt_surface* s, double x, double y, double z, double vx,
+                     double vy, double vz, double& dt, int niter, int start, double seed)
+{
+    if (newton_surface(s)) {
+        double d0 = seed;
+        bool ok = true;
+        if (start == BT_START_CONIC) ok = conic_start(s, x, y, z, vx, vy, vz, d0);
+        if (ok && d0 == d0 && newton_fast(s, x, y, z, vx, vy, vz, d0)) {
+            dt = d0;
+            return true;
+        }
+    }
+    dt = 0.0;
+    return surf_tti(s, x, y, z, vx, vy, vz, dt, niter);
+}
+
 // ---------------------------------------------------------------------------
 // Media (src/medium.cpp)
 // ---------------------------------------------------------------------------
@@ -616,7 +689,8 @@ struct Ray {
 BT_DEV bool apply_interaction(int action, const double* __restrict__ dr,
                               const double* __restrict__ drot,
                               const bt_surface* surf, double n2,
-                              Ray& r, int niter)
+                              Ray& r, int niter, int start = BT_START_REFERENCE,
+                              double seed = 0.0)
 {
     if (r.fail) return false;
     double dx = r.x-dr[0];
@@ -630,7 +704,10 @@ BT_DEV bool apply_interaction(int action, const double* __restrict__ dr,
     double vz = r.vx*drot[2] + r.vy*drot[5] + r.vz*drot[8];
     double t = r.t;
     double dt = 0.0;
-    if (!surf_tti(surf, x, y, z, vx, vy, vz, dt, niter)) {
+    const bool hit = (start == BT_START_REFERENCE)
+                         ? surf_tti(surf, x, y, z, vx, vy, vz, dt, niter)
+                         : fast_tti(surf, x, y, z, vx, vy, vz, dt, niter, start, seed);
+    if (!hit) {
         r.fail = true;
         r.vig = true;
         return false;          // left in the previous frame

@@ -1,0 +1,120 @@
+// bt_mlp.h -- the Newton-start network: 6 -> 64 -> 64 -> 1, ReLU.
+//
+// The network predicts, for a ray meeting a Newton surface, the residual
+//   delta = dt* - dt_conic
+// between the exact intersection time and the analytic root of the surface's
+// base conic, from the ray's point on that conic and its direction (all in
+// the surface's local frame).  It is trained in FP32 (bt_mlp.hip); inside the
+// trace it is evaluated wave-cooperatively on MFMA: the two hidden layers as
+// v_mfma_f32_32x32x16_f16 products with FP32 accumulation, laid out so each
+// layer's accumulator registers are the next layer's B operand directly.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace bt {
+
+constexpr int kMlpIn = 6;
+constexpr int kMlpHidden = 64;
+// FP32 training parameter vector: W1[64][6], b1[64], W2[64][64], b2[64],
+// w3[64], b3
+constexpr int kMlpOffW1 = 0;
+constexpr int kMlpOffB1 = kMlpOffW1 + kMlpHidden * kMlpIn;
+constexpr int kMlpOffW2 = kMlpOffB1 + kMlpHidden;
+constexpr int kMlpOffB2 = kMlpOffW2 + kMlpHidden * kMlpHidden;
+constexpr int kMlpOffW3 = kMlpOffB2 + kMlpHidden;
+constexpr int kMlpOffB3 = kMlpOffW3 + kMlpHidden;
+constexpr int kMlpParams = kMlpOffB3 + 1;
+
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+typedef float f16v __attribute__((ext_vector_type(16)));
+
+// Device inference tables (built by bt_mlp_pack from the FP32 parameters):
+//   norm: in_mean[6], in_inv_std[6], out_mean, out_std, b3, pad   (16 floats)
+//   w1:   [2 rb][64 lanes][8] half  = [W1[n][0..5], b1[n], 0], n = 32 rb + lane%32
+//   w2:   [2 rb2][4 s][64 lanes][8] half = W2[o][sigma(s, lane/32, e)],
+//         o = 32 rb2 + lane%32, sigma(2 rb1 + t, h, e) = 32 rb1 + 16 t + 8 (e/4) + 4 h + e%4
+//   b2:   [2 rb2][64 lanes][16] float = b2[32 rb2 + 8 (e/4) + 4 (lane/32) + e%4]
+//   w3:   [2 rb2][64 lanes][16] float, same row map as b2
+struct MlpPacked {
+    float norm[16];
+    _Float16 w1[2 * 64 * 8];
+    _Float16 w2[2 * 4 * 64 * 8];
+    float b2[2 * 64 * 16];
+    float w3[2 * 64 * 16];
+};
+
+// Row of element e of a 32x32 MFMA accumulator held by lane half h.
+__device__ __forceinline__ int acc_row(int e, int h) { return 8 * (e >> 2) + 4 * h + (e & 3); }
+
+// Evaluate the network for the 64 rays of the calling wave (one per lane).
+// Must be reached by all 64 lanes together (uniform control flow): MFMA
+// operates on the whole wave.  `f` are the lane's raw features; returns the
+// de-normalised prediction for the lane's own ray.
+__device__ __forceinline__ float mlp_predict_wave(const MlpPacked* __restrict__ P,
+                                                  const float (&f)[kMlpIn])
+{
+    const int lane = threadIdx.x & 63;
+    const int h = lane >> 5;
+    // normalised, augmented input (bias slot = 1) as 8 halves
+    h8 x;
+    for (int k = 0; k < kMlpIn; k++)
+        x[k] = (_Float16)((f[k] - P->norm[k]) * P->norm[6 + k]);
+    x[6] = (_Float16)1.0f;
+    x[7] = (_Float16)0.0f;
+    h8 zero;
+    for (int k = 0; k < 8; k++) zero[k] = (_Float16)0.0f;
+    // ray column block cb (the rays of lane half cb), one at a time to keep
+    // few accumulators live
+    float part[2] = {0.f, 0.f};
+    for (int cb = 0; cb < 2; cb++) {
+        // Reload the weights for each column block rather than holding all of
+        // them across both: an opaque (zero) byte offset keeps the compiler
+        // from hoisting the loads out of this loop.
+        int off;
+        asm volatile("s_mov_b32 %0, 0" : "=s"(off));
+        P = reinterpret_cast<const MlpPacked*>(reinterpret_cast<const char*>(P) + off);
+        const h8 b1 = (h == cb) ? x : zero;
+        // layer 1: H1^T[64 neurons][32 rays] as two 32x32 row blocks; relu(H1)
+        // in the accumulator layout is the layer-2 B operand
+        h8 hb[2][2];   // [rb1][t]
+        for (int rb = 0; rb < 2; rb++) {
+            const h8 a = *reinterpret_cast<const h8*>(P->w1 + (rb * 64 + lane) * 8);
+            f16v acc;
+            for (int e = 0; e < 16; e++) acc[e] = 0.f;
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b1, acc, 0, 0, 0);
+            for (int t = 0; t < 2; t++)
+                for (int e = 0; e < 8; e++) {
+                    const float v = acc[8 * t + e];
+                    hb[rb][t][e] = (_Float16)(v > 0.f ? v : 0.f);
+                }
+        }
+        // layers 2 and 3, one output row block at a time
+        for (int rb2 = 0; rb2 < 2; rb2++) {
+            asm volatile("s_mov_b32 %0, 0" : "=s"(off));
+            P = reinterpret_cast<const MlpPacked*>(reinterpret_cast<const char*>(P) + off);
+            f16v acc;
+            const float* b2 = P->b2 + (rb2 * 64 + lane) * 16;
+            for (int e = 0; e < 16; e++) acc[e] = b2[e];
+            for (int s = 0; s < 4; s++) {
+                const h8 a = *reinterpret_cast<const h8*>(P->w2 + ((rb2 * 4 + s) * 64 + lane) * 8);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, hb[s >> 1][s & 1], acc, 0, 0, 0);
+            }
+            const float* w3 = P->w3 + (rb2 * 64 + lane) * 16;
+            float p = 0.f;
+            for (int e = 0; e < 16; e++) {
+                const float v = acc[e];
+                p = __builtin_fmaf(w3[e], v > 0.f ? v : 0.f, p);
+            }
+            part[cb] += p;
+        }
+    }
+    // the two lane halves hold the two halves of each column's output rows
+    const float y0 = part[0] + __shfl_xor(part[0], 32);
+    const float y1 = part[1] + __shfl_xor(part[1], 32);
+    const float y = ((h == 0) ? y0 : y1) + P->norm[14];
+    return y * P->norm[13] + P->norm[12];
+}
+
+}  // namespace bt

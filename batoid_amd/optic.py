@@ -251,25 +251,34 @@ class CompoundOptic(Optic):
         visit(self)
         return tuple(flags)
 
-    def _plan(self, coordSys_in, reverse):
-        from .plan import PlanSpec, DevicePlan, collect_steps
-        key = (coordSys_in, reverse, self._skips())
+    def _plan(self, coordSys_in, reverse, start=None):
+        from .plan import PlanSpec, DevicePlan, collect_steps, start_key
+        key = (coordSys_in, reverse, self._skips(), start_key(start))
         cache = self.__dict__.setdefault('_plan_cache', {})
         plan = cache.get(key)
         if plan is None:
-            plan = DevicePlan(PlanSpec(collect_steps(self, reverse), coordSys_in))
+            plan = DevicePlan(PlanSpec(collect_steps(self, reverse), coordSys_in, start))
             cache[key] = plan
         return plan
 
-    def trace(self, rv, reverse=False, path=None):
+    def trace(self, rv, reverse=False, path=None, start=None):
         """Trace rays through every unskipped item, in place
         (optic.py:1013-1083).  Without ``path`` the whole optic runs as one
         fused GPU kernel; with ``path`` the items are traced one by one in
-        the requested order."""
+        the requested order.
+
+        ``start`` picks how Newton surfaces seed their intersection:
+        "reference" (default; bit-identical to the reference), "conic" (the
+        base conic's root, converged to a 1e-13 m sag gap), or a mapping
+        {interface name: NewtonStartNet} (learned starts, see
+        newton_start.py).  Fast starts fall back to the reference path for
+        any ray they do not converge."""
         from .global_vars import _batoid_niter
         if path is not None:
+            if start not in (None, "reference"):
+                raise NotImplementedError("fast starts apply to the fused trace only")
             return self._trace_path(rv, path)
-        plan = self._plan(rv.coordSys, reverse)
+        plan = self._plan(rv.coordSys, reverse, start)
         if plan.struct.nstep == 0:
             return rv
         rays = rv._rays

@@ -25,10 +25,42 @@ def _key(obj):
         return (type(obj), id(obj))
 
 
+def _start_of(start, iface):
+    """(bt_start, packed network or None) of one step.
+
+    ``start`` is "reference" (or None), "conic", or a mapping from
+    Interface name to a trained NewtonStartNet; in a mapping, Newton
+    surfaces without a network use the conic start."""
+    from .newton_start import _is_newton
+    if start is None or start == "reference":
+        return _lib.START_REFERENCE, None
+    if not _is_newton(iface.surface):
+        return _lib.START_REFERENCE, None
+    if start == "conic":
+        return _lib.START_CONIC, None
+    if isinstance(start, dict):
+        net = start.get(iface.name)
+        if net is None:
+            return _lib.START_CONIC, None
+        return _lib.START_MLP, net.packed
+    raise ValueError(f"unknown start {start!r}: use 'reference', 'conic' or "
+                     "a {name: NewtonStartNet} mapping")
+
+
+def start_key(start):
+    """Hashable identity of a start argument (for plan caches)."""
+    if start is None or start == "reference":
+        return "reference"
+    if isinstance(start, dict):
+        # the packed buffer changes whenever a network is re-fit
+        return tuple(sorted((k, v.packed.data_ptr()) for k, v in start.items()))
+    return start
+
+
 class PlanSpec:
     """Host-side plan: ctypes records plus the numpy tables they point to."""
 
-    def __init__(self, steps, coordSys_in):
+    def __init__(self, steps, coordSys_in, start=None):
         self.surf_recs, self.surf_arrs = [], []
         self.med_recs, self.med_arrs = [], []
         self.ob_recs, self.ob_arrs = [], []
@@ -36,6 +68,8 @@ class PlanSpec:
         self.interfaces = []
         surf_index, med_index, ob_index = {}, {}, {}
         refracts = False
+        fast = False
+        self.networks = []      # packed network buffers the steps point to
         cs = coordSys_in
         for iface, action, m2 in steps:
             key = _key(iface.surface)
@@ -70,6 +104,12 @@ class PlanSpec:
             st.medium = mi
             st.obsc = oi
             st.obsc_len = ol
+            mode, packed = _start_of(start, iface)
+            st.start = mode
+            if packed is not None:
+                self.networks.append(packed)
+                st.mlp = packed.data_ptr()
+            fast = fast or mode != _lib.START_REFERENCE
             for i, v in enumerate(ct.dr):
                 st.dr[i] = float(v)
             for i, v in enumerate(np.asarray(ct.drot).ravel()):
@@ -79,7 +119,8 @@ class PlanSpec:
             cs = iface.coordSys
         self.coordSys_in = coordSys_in
         self.coordSys_out = cs
-        self.flags = 0 if refracts else _lib.PLAN_NO_REFRACT
+        self.flags = (0 if refracts else _lib.PLAN_NO_REFRACT) | (
+            _lib.PLAN_FAST_START if fast else 0)
 
 
 class DevicePlan:

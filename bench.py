@@ -151,6 +151,72 @@ def bench_huygens(optic, reps=10):
     return res
 
 
+def bench_newton_start(nrad, naz, reps=5, train_steps=3000):
+    """The perturbed TMA (every mirror Sum(Asphere, Zernike j<=28, Bicubic
+    204^2)) traced with the reference start, the conic start and learned
+    starts.  The networks are trained first on 12 other fields (recorded
+    with their wall time); each mode is then timed on one 1e7-ray batch and
+    checked against the reference-start trace (max |dx|, flags)."""
+    import torch
+    from batoid_amd import workloads
+    from batoid_amd.newton_start import train_newton_starts
+    optic = workloads.perturbed_tma()
+    thx, thy = workloads.tma_fields()
+    t0 = time.perf_counter()
+    train_rvs = [workloads.tma_rays(optic, thx[k], thy[k], 620e-9, nrad=60, naz=600)
+                 for k in range(6, 18)]
+    nets = train_newton_starts(optic, train_rvs, steps=train_steps, batch=8192)
+    torch.cuda.synchronize()
+    train_s = time.perf_counter() - t0
+    log(f"newton-start networks trained in {train_s:.1f} s: "
+        + ", ".join(f"{k} loss {v.history[-1][1]:.2e}" for k, v in nets.items()))
+    rv0 = workloads.tma_rays(optic, thx[0], thy[0], 620e-9, nrad=nrad, naz=naz)
+    n = len(rv0)
+    nsteps = optic._plan(rv0.coordSys, False).struct.nstep
+    res = {"workload": "perturbed TMA (Sum(Asphere, Zernike j<=28 N(0,50nm), "
+                       "Bicubic 204^2 10nm rms) per mirror), field 0, 620 nm",
+           "rays": n, "interfaces": nsteps, "train_seconds": train_s,
+           "train_samples_per_surface": 12 * len(train_rvs[0]),
+           "train_steps": train_steps}
+    ref_cols = None
+    for mode in ("reference", "conic", "mlp"):
+        start = nets if mode == "mlp" else mode
+        rv = rv0.copy()
+        optic.trace(rv, start=start)            # plan build + warm-up
+        torch.cuda.synchronize()
+        cols = {k: rv.torch(k).clone() for k in ("x", "y", "z", "vx", "vy", "vz", "t",
+                                                "vignetted", "failed")}
+        work = [rv0.copy() for _ in range(reps)]
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record()
+        for w in work:
+            optic.trace(w, start=start)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        entry = {"ms_per_trace": ms, "ray_surfaces_per_s": n * nsteps / (ms * 1e-3)}
+        if ref_cols is None:
+            ref_cols = cols
+        else:
+            ok = ref_cols["failed"] == 0
+            entry["max_abs_dpos_m"] = max(float((cols[k] - ref_cols[k])[ok].abs().max())
+                                          for k in ("x", "y", "z"))
+            entry["max_abs_ddir"] = max(float((cols[k] - ref_cols[k])[ok].abs().max())
+                                        for k in ("vx", "vy", "vz"))
+            entry["flag_mismatches"] = int((cols["failed"] != ref_cols["failed"]).sum() +
+                                           (cols["vignetted"] != ref_cols["vignetted"]).sum())
+            entry["speedup_vs_reference_start"] = res["reference"]["ms_per_trace"] / ms
+        res[mode] = entry
+        log(f"newton start {mode}: {ms:.3f} ms/trace, "
+            f"{entry['ray_surfaces_per_s']:.3e} ray-surfaces/s"
+            + (f", max |dpos| {entry['max_abs_dpos_m']:.2e} m, flags differ "
+               f"{entry['flag_mismatches']}" if "flag_mismatches" in entry else ""))
+        del work
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -161,6 +227,7 @@ def main():
     ap.add_argument("--naz", type=int, default=11080)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-huygens", action="store_true")
+    ap.add_argument("--no-newton-start", action="store_true")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="also run a plain device copy of this many MiB (calibrates "
                          "FETCH_SIZE / WRITE_SIZE counters against known bytes)")
@@ -282,6 +349,8 @@ def main():
     }
     if not args.no_huygens:
         result["huygens"] = bench_huygens(optic)
+    if not args.no_newton_start:
+        result["newton_start"] = bench_newton_start(args.nrad, args.naz)
     if not args.no_cpu:
         log("timing the CPU baseline ...")
         result["cpu_baseline"] = cpu_baseline(optic, batches[0])

@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define BT_ABI_VERSION 1
+#define BT_ABI_VERSION 2
 
 /* ---- status codes ------------------------------------------------------ */
 #define BT_OK            0
@@ -151,19 +151,36 @@ typedef struct bt_obsc {
  * the interaction, the surface, the outgoing medium and the obscuration.  */
 enum bt_action { BT_ACT_INTERSECT = 0, BT_ACT_REFLECT = 1, BT_ACT_REFRACT = 2 };
 
+/* How a step seeds the Newton intersection of a Newton surface (Asphere,
+ * Bicubic, Zernike/Polynomial, Sum):
+ *   REFERENCE  the reference's start and exactly niter iterations
+ *              (surface.cpp:22-49, asphere.cpp:75-85, sum.cpp:47-56):
+ *              bit-identical to the reference;
+ *   CONIC      the analytic root of the base conic (Asphere's quadric, or a
+ *              Sum's first child), then Newton on the full surface until the
+ *              sag gap is <= 1e-13 m (at most 8 iterations);
+ *   MLP        as CONIC, plus the learned correction of the step's network.
+ * A ray that does not converge from a fast start is re-intersected on the
+ * REFERENCE path, so failure semantics stay the reference's.  Starts other
+ * than REFERENCE are honoured only by bt_trace_plan with BT_PLAN_FAST_START
+ * set; an MLP step with a null network runs as CONIC. */
+enum bt_start { BT_START_REFERENCE = 0, BT_START_CONIC = 1, BT_START_MLP = 2 };
+
 typedef struct bt_step {
     int32_t action;
     int32_t surface;    /* index of the first record in the surface table   */
     int32_t medium;     /* index in the medium table (REFRACT), else -1     */
     int32_t obsc;       /* index of the first obscuration record, or -1     */
     int32_t obsc_len;   /* number of obscuration records                    */
-    int32_t pad_;
+    int32_t start;      /* bt_start                                         */
     double dr[3];
     double drot[9];
+    const void* mlp;    /* packed network (bt_mlp_pack) for BT_START_MLP    */
 } bt_step;
 
 /* bt_plan.flags */
 #define BT_PLAN_NO_REFRACT 1   /* no step refracts: wavelength is not read */
+#define BT_PLAN_FAST_START 2   /* some step uses a CONIC or MLP start      */
 
 typedef struct bt_plan {
     int32_t nstep;
@@ -267,6 +284,42 @@ int64_t bt_sum_amplitude_workspace_bytes(int64_t n);
 int bt_sum_amplitude(bt_rays rays, double x, double y, double z, double t,
                      int ignore_vignetted, double* out, void* workspace,
                      void* stream);
+
+/* ---- learned Newton start ------------------------------------------------
+ * No counterpart in the reference: the network that seeds BT_START_MLP
+ * steps, 6 -> 64 -> 64 -> 1 with ReLU, trained in FP32 on intersections
+ * the tracer produces.  Parameters are one FP32 vector of
+ * bt_mlp_param_count() floats: W1[64][6], b1[64], W2[64][64], b2[64],
+ * w3[64], b3.  `in_norm` is in_mean[6], in_inv_std[6] (14 floats with
+ * out_mean, out_std for bt_mlp_pack).  Features are [n][6] FP32: the ray's
+ * point on the surface's base conic and its direction, local frame; the
+ * target is (dt_exact - dt_conic - out_mean) / out_std.                     */
+int bt_mlp_param_count(void);
+int64_t bt_mlp_packed_bytes(void);
+int64_t bt_mlp_grad_workspace_bytes(int64_t batch);
+/* Mean squared error over `batch` samples (rows index[b] of features /
+ * targets, or rows 0..batch-1 when index is NULL) and its gradient. */
+int bt_mlp_loss_grad(const float* params, const float* features, const float* targets,
+                     const int32_t* index, int64_t batch, const float* in_norm,
+                     float* grad, float* loss, void* workspace, int64_t workspace_bytes,
+                     void* stream);
+int bt_mlp_adam(float* params, const float* grad, float* m, float* v, int64_t nparam,
+                float lr, float beta1, float beta2, float eps, int step, void* stream);
+/* FP32 inference, output in normalised units. */
+int bt_mlp_forward(const float* params, const float* in_norm, const float* features,
+                   int64_t n, float* out, void* stream);
+/* Pack FP32 parameters + norm[14] into the fp16 MFMA tables used in the
+ * trace (bt_mlp_packed_bytes() bytes at `packed`). */
+int bt_mlp_pack(const float* params, const float* norm, void* packed, void* stream);
+/* Packed (MFMA) inference, output de-normalised (metres). */
+int bt_mlp_forward_packed(const void* packed, const float* features, int64_t n,
+                          float* out, void* stream);
+/* Training samples from the tracer: for each ray, the local frame of the
+ * step, the base-conic start, the reference intersection (niter Newton
+ * iterations) and status bits (1: conic start ok, 2: intersection ok). */
+int bt_newton_samples(const bt_surface* surface, const double* dr, const double* drot,
+                      bt_rays rays, int niter, float* features, double* dt_conic,
+                      double* dt_exact, uint8_t* status, void* stream);
 
 /* ---- Huygens PSF ---------------------------------------------------------
  * The reference evaluates the image of huygensPSF (analysis.py:181-305) as a

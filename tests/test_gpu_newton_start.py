@@ -1,0 +1,213 @@
+"""Learned Newton starts: the training kernels against the FP64 numpy
+restatement (oracle/mlp.py), the packed MFMA inference against the FP32
+inference, the tracer's sampler against the reference intersection, and the
+fast-start traces against the reference-start trace (positions and times
+within 1e-12 m, directions within 1e-14, flags identical), plus the gain the
+network brings over the conic start."""
+import ctypes
+
+import numpy as np
+import pytest
+import torch
+
+import batoid_amd as bt
+from batoid_amd import _lib, _device, workloads
+from batoid_amd.newton_start import (NewtonStartNet, newton_samples, collect_samples,
+                                     train_newton_starts)
+from oracle import mlp as omlp
+
+pytestmark = pytest.mark.gpu
+
+FIELDS_TRAIN = range(0, 12)
+FIELDS_TEST = (100, 150, 188)
+
+
+def _p(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _random_problem(n, seed):
+    rng = np.random.default_rng(seed)
+    net = NewtonStartNet(seed=seed)
+    # non-trivial biases and normalisation so every term of the gradient runs
+    p = net.params.cpu().numpy().astype(np.float64)
+    p[omlp.OFF_B1:omlp.OFF_W2] = rng.normal(0, 0.3, omlp.N_HID)
+    p[omlp.OFF_B2:omlp.OFF_W3] = rng.normal(0, 0.3, omlp.N_HID)
+    p[omlp.OFF_B3] = 0.1
+    net.params = torch.from_numpy(p.astype(np.float32)).cuda()
+    feat = rng.normal(0, 1, (n, 6)) * [2.0, 2.0, 0.3, 0.1, 0.1, 1.0] + [0.1, -0.2, 0.05, 0, 0, -1]
+    net.in_norm = np.concatenate([[0.1, -0.2, 0.05, 0, 0, -1],
+                                  1 / np.array([2.0, 2.0, 0.3, 0.1, 0.1, 1.0])]).astype(np.float32)
+    targets = rng.normal(0, 1, n)
+    return net, feat.astype(np.float32), targets.astype(np.float32)
+
+
+def test_loss_grad_matches_oracle():
+    net, feat, targ = _random_problem(5000, seed=3)
+    p64 = net.params.cpu().numpy().astype(np.float64)
+    loss_o, grad_o = omlp.loss_grad(p64, feat.astype(np.float64), targ, net.in_norm)
+    f = torch.from_numpy(feat).cuda()
+    t = torch.from_numpy(targ).cuda()
+    loss, grad = net.loss_grad(f, t)
+    torch.cuda.synchronize()
+    g = grad.cpu().numpy().astype(np.float64)
+    scale = np.abs(grad_o).max()
+    assert abs(loss.item() - loss_o) <= 1e-5 * loss_o
+    assert np.abs(g - grad_o).max() <= 2e-5 * scale, np.abs(g - grad_o).max() / scale
+    # every block of the parameter vector is exercised
+    for a, b in ((omlp.OFF_W1, omlp.OFF_B1), (omlp.OFF_B1, omlp.OFF_W2), (omlp.OFF_W2, omlp.OFF_B2),
+                 (omlp.OFF_B2, omlp.OFF_W3), (omlp.OFF_W3, omlp.OFF_B3)):
+        assert np.abs(grad_o[a:b]).max() > 0
+
+
+def test_loss_grad_with_index_matches_oracle():
+    net, feat, targ = _random_problem(3000, seed=4)
+    rng = np.random.default_rng(9)
+    idx = rng.integers(0, 3000, 777).astype(np.int32)   # ragged: not a multiple of 64
+    p64 = net.params.cpu().numpy().astype(np.float64)
+    loss_o, grad_o = omlp.loss_grad(p64, feat[idx].astype(np.float64), targ[idx], net.in_norm)
+    loss, grad = net.loss_grad(torch.from_numpy(feat).cuda(), torch.from_numpy(targ).cuda(),
+                               torch.from_numpy(idx).cuda())
+    torch.cuda.synchronize()
+    g = grad.cpu().numpy().astype(np.float64)
+    assert abs(loss.item() - loss_o) <= 1e-5 * loss_o
+    assert np.abs(g - grad_o).max() <= 2e-5 * np.abs(grad_o).max()
+
+
+def test_adam_matches_oracle():
+    rng = np.random.default_rng(5)
+    n = omlp.N_PARAMS
+    # FP32 inputs, so the FP64 oracle sees exactly what the kernel sees
+    p, g = rng.normal(0, 1, n).astype(np.float32), rng.normal(0, 1e-2, n).astype(np.float32)
+    m = rng.normal(0, 1e-3, n).astype(np.float32)
+    v = rng.uniform(0, 1e-4, n).astype(np.float32)
+    dev = [torch.from_numpy(a.astype(np.float32)).cuda() for a in (p, g, m, v)]
+    _lib.call("bt_mlp_adam", *[_p(t) for t in dev], n, 1e-3, 0.9, 0.999, 1e-8, 7,
+              _device.stream())
+    torch.cuda.synchronize()
+    # hyper-parameters as the kernel holds them (FP32)
+    f32 = lambda x: float(np.float32(x))
+    po, mo, vo = omlp.adam(*(a.astype(np.float64) for a in (p, g, m, v)), 7, f32(1e-3),
+                           beta1=f32(0.9), beta2=f32(0.999), eps=f32(1e-8))
+    # FP32 arithmetic: a few units in the last place of each result's scale
+    for got, want in ((dev[0], po), (dev[2], mo), (dev[3], vo)):
+        ulp = np.spacing(np.float32(np.abs(want).max()))
+        assert np.abs(got.cpu().numpy() - want).max() <= 4 * ulp
+
+
+def test_forward_fp32_and_packed_match_oracle():
+    net, feat, _ = _random_problem(10_001, seed=6)     # ragged last wave
+    net.out_mean, net.out_std = 3e-7, 2e-7
+    y_o = omlp.forward(net.params.cpu().numpy(), feat, net.in_norm) * 2e-7 + 3e-7
+    f = torch.from_numpy(feat).cuda()
+    y32 = net.predict(f).cpu().numpy()
+    y16 = net.predict_packed(f).cpu().numpy().astype(np.float64)
+    spread = np.std(y_o)
+    assert np.abs(y32 - y_o).max() <= 1e-5 * np.abs(y_o).max()
+    # fp16 weights and activations with FP32 accumulation
+    err = y16 - y_o
+    assert np.sqrt(np.mean(err ** 2)) <= 3e-3 * spread, np.sqrt(np.mean(err ** 2)) / spread
+    assert np.abs(err).max() <= 2e-2 * spread
+
+
+def test_samples_match_reference_intersection():
+    optic = workloads.perturbed_tma()
+    thx, thy = workloads.tma_fields()
+    rv = workloads.tma_rays(optic, thx[3], thy[3], 620e-9, nrad=30, naz=300)
+    m1 = optic["M1"]
+    feat, dtc, dte, status = newton_samples(m1.surface, m1.coordSys, rv)
+    ref = rv.copy()
+    bt.intersect(m1.surface, ref, m1.coordSys)
+    torch.cuda.synchronize()
+    st = status.cpu().numpy()
+    ok = st == 3
+    assert ok.mean() > 0.99
+    assert np.array_equal(ref.failed, (st & 2) == 0)
+    # the sampler's dt* reproduces the reference intersection point exactly
+    loc = rv.copy().toCoordSys(m1.coordSys)
+    x = loc.x + loc.vx * dte.cpu().numpy()
+    np.testing.assert_array_equal(x[ok], ref.x[ok])
+    # the features are the conic point and the direction
+    f = feat.cpu().numpy().astype(np.float64)
+    xc = loc.x + loc.vx * dtc.cpu().numpy()
+    np.testing.assert_allclose(f[ok, 0], xc[ok], rtol=1e-7, atol=1e-7)
+    np.testing.assert_allclose(f[ok, 5], loc.vz[ok], rtol=1e-7)
+
+
+@pytest.fixture(scope="module")
+def trained():
+    optic = workloads.perturbed_tma()
+    thx, thy = workloads.tma_fields()
+    rvs = [workloads.tma_rays(optic, thx[k], thy[k], 620e-9, nrad=40, naz=400)
+           for k in FIELDS_TRAIN]
+    nets = train_newton_starts(optic, rvs, steps=2500, batch=4096)
+    return optic, nets
+
+
+def test_training_converges_and_predicts_residual(trained):
+    optic, nets = trained
+    # the three perturbed mirrors and the aspheric exit face of L2
+    assert sorted(nets) == ["L2_exit", "M1", "M2", "M3"]
+    thx, thy = workloads.tma_fields()
+    rvs = [workloads.tma_rays(optic, thx[k], thy[k], 620e-9, nrad=30, naz=300)
+           for k in FIELDS_TEST]
+    held = collect_samples(optic, rvs)
+    for name, net in nets.items():
+        hist = [l for _, l in net.history]
+        assert hist[-1] < 0.05 * hist[0], (name, hist)
+        feat, delta = held[name]
+        d = delta.cpu().numpy()
+        pred = net.predict_packed(feat).cpu().numpy().astype(np.float64)
+        resid = pred - d
+        gain = np.sqrt(np.mean(d ** 2)) / np.sqrt(np.mean(resid ** 2))
+        # the learned start is >= 10x closer to the root than the conic one
+        assert gain >= 10, (name, gain)
+
+
+def _cols(rv):
+    torch.cuda.synchronize()
+    return {k: getattr(rv, k) for k in ("x", "y", "z", "vx", "vy", "vz", "t",
+                                        "vignetted", "failed")}
+
+
+@pytest.mark.parametrize("mode", ["conic", "mlp"])
+def test_fast_start_trace_matches_reference(trained, mode):
+    optic, nets = trained
+    thx, thy = workloads.tma_fields()
+    start = nets if mode == "mlp" else "conic"
+    for k in FIELDS_TEST:
+        for wl in (350e-9, 1050e-9):
+            rv = workloads.tma_rays(optic, thx[k], thy[k], wl, nrad=50, naz=500)
+            ref = rv.copy()
+            optic.trace(ref)
+            fast = rv.copy()
+            optic.trace(fast, start=start)
+            a, b = _cols(ref), _cols(fast)
+            assert np.array_equal(a["failed"], b["failed"])
+            assert np.array_equal(a["vignetted"], b["vignetted"])
+            ok = ~a["failed"]
+            assert ok.mean() > 0.99
+            for c in ("x", "y", "z", "t"):
+                assert np.abs(a[c][ok] - b[c][ok]).max() <= 1e-12, (mode, k, wl, c)
+            for c in ("vx", "vy", "vz"):
+                assert np.abs(a[c][ok] - b[c][ok]).max() <= 1e-14, (mode, k, wl, c)
+
+
+def test_fast_start_unperturbed_and_ragged(trained):
+    """Ragged sizes (lanes past the end in the last wave) and a plan whose
+    networks cover only some surfaces (the others run the conic start)."""
+    optic, nets = trained
+    thx, thy = workloads.tma_fields()
+    base = workloads.tma_rays(optic, thx[50], thy[50], 620e-9, nrad=13, naz=101)
+    for n in (1, 63, 65, 1000, len(base)):
+        rv = base[:n] if n < len(base) else base
+        ref = rv.copy()
+        optic.trace(ref)
+        fast = rv.copy()
+        optic.trace(fast, start={"M2": nets["M2"]})
+        a, b = _cols(ref), _cols(fast)
+        assert np.array_equal(a["failed"], b["failed"])
+        assert np.array_equal(a["vignetted"], b["vignetted"])
+        ok = ~a["failed"]
+        for c in ("x", "y", "z", "t"):
+            assert np.abs(a[c][ok] - b[c][ok]).max(initial=0) <= 1e-12
