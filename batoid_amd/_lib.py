@@ -69,7 +69,15 @@ class BtPlan(ctypes.Structure):
     ]
 
 
-STRUCTS = (BtRays, BtSurface, BtMedium, BtObsc, BtStep, BtPlan)
+class BtPsfSegment(ctypes.Structure):
+    _fields_ = [
+        ("begin", ctypes.c_int64), ("end", ctypes.c_int64),
+        ("chief", ctypes.c_int64), ("time_ray", ctypes.c_int64),
+        ("a1", ctypes.c_double * 2), ("a2", ctypes.c_double * 2),
+    ]
+
+
+STRUCTS = (BtRays, BtSurface, BtMedium, BtObsc, BtStep, BtPlan, BtPsfSegment)
 
 # enum values (include/batoid_amd.h)
 PLANE, SPHERE, PARABOLOID, QUADRIC, ASPHERE, TILTED, BICUBIC, POLYNOMIAL, SUM = range(9)
@@ -127,6 +135,8 @@ SIGNATURES = {
     "bt_huygens_workspace_bytes": (_L, [_L, _I, _I]),
     "bt_huygens_direct": (_I, [BtRays, _V, _V, _V, _I, _I, _D, _I, _V, _V, _L, _V]),
     "bt_huygens_gemm": (_I, [BtRays, _V, _V, _V, _I, _I, _D, _I, _I, _V, _V, _L, _V]),
+    "bt_huygens_batch_workspace_bytes": (_L, [_L, _I, _I, _I, _I]),
+    "bt_huygens_batch": (_I, [BtRays, _V, _I, _I, _I, _I, _I, _I, _I, _V, _V, _L, _V]),
 }
 
 

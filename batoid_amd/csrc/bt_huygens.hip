@@ -108,8 +108,43 @@ __global__ __launch_bounds__(kScanThreads) void k_psf_scan(const int* __restrict
     if (threadIdx.x == kScanThreads - 1) offsets[nb] = s[kScanThreads - 1];
 }
 
-// Per ray: phase at p0 and per-pixel phase steps, written compacted in ray
-// order: [alpha_hi, alpha_lo, beta_hi, beta_lo, phi0_rev, flux, 0, 0].
+// Per ray: phase at p0 (revolutions) and per-pixel phase steps
+// [alpha_hi, alpha_lo, beta_hi, beta_lo, phi0_rev, flux, 0, 0].
+__device__ __forceinline__ void prep_ray(const bt_rays& rv, int64_t i, double p0x, double p0y,
+                                         double tref, double a1x, double a1y, double a2x,
+                                         double a2y, int keep_a, int keep_b,
+                                         float* __restrict__ o)
+{
+    const double x = rv.x[i], y = rv.y[i], z = rv.z[i];
+    const double vx = rv.vx[i], vy = rv.vy[i], vz = rv.vz[i];
+    const double t = rv.t[i], w_ = rv.wavelength[i];
+    // the reference's phase at p0 (rayVector.cpp:207-214), z_p = 0
+    const double v2 = vx*vx + vy*vy + vz*vz;
+    double ph = (p0x-x)*vx;
+    ph += (p0y-y)*vy;
+    ph += (0.0-z)*vz;
+    ph /= v2;
+    ph -= tref-t;
+    ph *= kTwoPi / w_;
+    double rev = ph / kTwoPi;
+    rev -= floor(rev);
+    // revolutions per pixel step along a1 and a2
+    const double inv = 1.0 / (v2 * w_);
+    const double al = (a1x*vx + a1y*vy) * inv;
+    const double be = (a2x*vx + a2y*vy) * inv;
+    const float ah = hi_bits((float)al, keep_a);
+    const float bh = hi_bits((float)be, keep_b);
+    o[0] = ah;
+    o[1] = (float)(al - (double)ah);
+    o[2] = bh;
+    o[3] = (float)(be - (double)bh);
+    o[4] = (float)rev;
+    o[5] = (float)rv.flux[i];
+    o[6] = 0.f;
+    o[7] = 0.f;
+}
+
+// The parameters of every valid ray, compacted in ray order.
 __global__ __launch_bounds__(kBlock) void k_psf_prep(bt_rays rv, PsfGeom g, int ignore_vig,
                                                       const int* __restrict__ offsets,
                                                       float* __restrict__ params)
@@ -125,35 +160,8 @@ __global__ __launch_bounds__(kBlock) void k_psf_prep(bt_rays rv, PsfGeom g, int 
     int base = offsets[blockIdx.x];
     for (int k = 0; k < w; k++) base += s[k];
     const int rank = base + __popcll(m & ((1ull << lane) - 1ull));
-
-    const double x = rv.x[i], y = rv.y[i], z = rv.z[i];
-    const double vx = rv.vx[i], vy = rv.vy[i], vz = rv.vz[i];
-    const double t = rv.t[i], w_ = rv.wavelength[i];
-    // the reference's phase at p0 (rayVector.cpp:207-214), z_p = 0
-    const double v2 = vx*vx + vy*vy + vz*vz;
-    double ph = (g.p0x-x)*vx;
-    ph += (g.p0y-y)*vy;
-    ph += (0.0-z)*vz;
-    ph /= v2;
-    ph -= g.t-t;
-    ph *= kTwoPi / w_;
-    double rev = ph / kTwoPi;
-    rev -= floor(rev);
-    // revolutions per pixel step along a1 and a2
-    const double inv = 1.0 / (v2 * w_);
-    const double al = (g.a1x*vx + g.a1y*vy) * inv;
-    const double be = (g.a2x*vx + g.a2y*vy) * inv;
-    const float ah = hi_bits((float)al, g.keep_a);
-    const float bh = hi_bits((float)be, g.keep_b);
-    float* o = params + (int64_t)rank * kRayStride;
-    o[0] = ah;
-    o[1] = (float)(al - (double)ah);
-    o[2] = bh;
-    o[3] = (float)(be - (double)bh);
-    o[4] = (float)rev;
-    o[5] = (float)rv.flux[i];
-    o[6] = 0.f;
-    o[7] = 0.f;
+    prep_ray(rv, i, g.p0x, g.p0y, g.t, g.a1x, g.a1y, g.a2x, g.a2y, g.keep_a, g.keep_b,
+             params + (int64_t)rank * kRayStride);
 }
 
 // sin(2 pi theta) with theta in revolutions (v_sin_f32).
@@ -166,25 +174,18 @@ __device__ __forceinline__ float step_phase(float c, float hi, float lo, float a
     return __builtin_amdgcn_fractf(c * hi) + __builtin_fmaf(c, lo, add);
 }
 
+// One workgroup's 128 x 128 complex tile of the sum over rays [r0, r1) of
+// the prepared parameters, written as FP32 (re, im) into out[i][j][2] (the
+// padded nxp x nyp image).  Every lane of the workgroup must call it.
 template <int kPrec>
-__global__ __launch_bounds__(256, 2) void k_psf_gemm(
-    const float* __restrict__ params, const int* __restrict__ total_ptr,
-    int nx, int ny, int tiles_i, int ksplit, int nxp, int nyp,
-    float* __restrict__ partial)
+__device__ __forceinline__ void psf_tile(const float* __restrict__ params, int r0, int r1,
+                                         int tile_i0, int tile_j0, int nx, int ny, int nyp,
+                                         float* __restrict__ out)
 {
-    const int tile = blockIdx.x;
-    const int ks = blockIdx.y;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const int i0 = (tile % tiles_i) * kTile + (wave & 1) * 64;
-    const int j0 = (tile / tiles_i) * kTile + (wave >> 1) * 64;
-
-    const int total = *total_ptr;
-    int per = (total + ksplit - 1) / ksplit;
-    per = ((per + kRayGroup - 1) / kRayGroup) * kRayGroup;
-    const int r0 = ks * per;
-    const int r1 = min(total, r0 + per);
-
+    const int i0 = tile_i0 + (wave & 1) * 64;
+    const int j0 = tile_j0 + (wave >> 1) * 64;
     const int half = lane >> 5;       // k half of the operand
     const int l32 = lane & 31;
     v16f acc[4][2];
@@ -281,8 +282,7 @@ __global__ __launch_bounds__(256, 2) void k_psf_gemm(
     }
 
     // D[row][col]: row = 8*(e/4) + 4*half + e%4, col = l32 (32x32 MFMA layout)
-    // partial[ks][i][j][2] over the padded nxp x nyp image
-    float* out = partial + (int64_t)ks * nxp * nyp * 2;
+    // out[i][j][2] over the padded nxp x nyp image
     for (int rb = 0; rb < 4; rb++)
         for (int cb = 0; cb < 2; cb++)
             for (int e = 0; e < 16; e++) {
@@ -291,6 +291,24 @@ __global__ __launch_bounds__(256, 2) void k_psf_gemm(
                 const int j = j0 + cb*32 + l32;
                 out[((int64_t)i * nyp + j) * 2 + (row & 1)] = acc[rb][cb][e];
             }
+}
+
+
+template <int kPrec>
+__global__ __launch_bounds__(256, 2) void k_psf_gemm(
+    const float* __restrict__ params, const int* __restrict__ total_ptr,
+    int nx, int ny, int tiles_i, int ksplit, int nxp, int nyp,
+    float* __restrict__ partial)
+{
+    const int tile = blockIdx.x;
+    const int ks = blockIdx.y;
+    const int total = *total_ptr;
+    int per = (total + ksplit - 1) / ksplit;
+    per = ((per + kRayGroup - 1) / kRayGroup) * kRayGroup;
+    const int r0 = ks * per;
+    const int r1 = min(total, r0 + per);
+    psf_tile<kPrec>(params, r0, r1, (tile % tiles_i) * kTile, (tile / tiles_i) * kTile, nx, ny,
+                    nyp, partial + (int64_t)ks * nxp * nyp * 2);
 }
 
 // amp[j][i] = sum_ks partial[ks][i][j] in FP64, fixed order.  Threads walk
@@ -311,6 +329,225 @@ __global__ __launch_bounds__(kBlock) void k_psf_reduce(const float* __restrict__
         im += (double)v.y;
     }
     const int64_t pix = (int64_t)j * nx + i;
+    amp[2*pix] = re;
+    amp[2*pix+1] = im;
+}
+
+// ---------------------------------------------------------------------------
+// Batched PSFs: many (field, wavelength) segments of one ray batch, each with
+// its own lattice, reference point and time, summed in one persistent GEMM
+// launch.  Capture-safe: no host synchronisation, allocation or memset.
+// ---------------------------------------------------------------------------
+struct SegGeom {
+    double p0x, p0y, t, pad;
+};
+
+// Per segment: reference point (the chief ray's position, or the mean of the
+// unvignetted rays' positions, analysis.py:289-295) and time (the t of
+// `time_ray`, analysis.py:301).  One block per segment; the mean is summed in
+// a fixed order.
+__global__ __launch_bounds__(kBlock) void k_psf_seg_geom(bt_rays rv,
+                                                          const bt_psf_segment* __restrict__ segs,
+                                                          SegGeom* __restrict__ geom)
+{
+    __shared__ double sx[kBlock], sy[kBlock];
+    __shared__ int64_t sc[kBlock];
+    const bt_psf_segment sg = segs[blockIdx.x];
+    const int64_t b = sg.begin < 0 ? 0 : (sg.begin > rv.n ? rv.n : sg.begin);
+    const int64_t e = sg.end < b ? b : (sg.end > rv.n ? rv.n : sg.end);
+    double px, py;
+    if (sg.chief >= 0 && sg.chief < rv.n) {
+        px = rv.x[sg.chief];
+        py = rv.y[sg.chief];
+    } else {
+        double ax = 0.0, ay = 0.0;
+        int64_t c = 0;
+        for (int64_t i = b + threadIdx.x; i < e; i += kBlock)
+            if (!rv.vignetted[i]) { ax += rv.x[i]; ay += rv.y[i]; c++; }
+        sx[threadIdx.x] = ax;
+        sy[threadIdx.x] = ay;
+        sc[threadIdx.x] = c;
+        __syncthreads();
+        for (int d = kBlock / 2; d > 0; d >>= 1) {
+            if ((int)threadIdx.x < d) {
+                sx[threadIdx.x] += sx[threadIdx.x + d];
+                sy[threadIdx.x] += sy[threadIdx.x + d];
+                sc[threadIdx.x] += sc[threadIdx.x + d];
+            }
+            __syncthreads();
+        }
+        px = sx[0] / (double)sc[0];
+        py = sy[0] / (double)sc[0];
+    }
+    if (threadIdx.x == 0) {
+        const int64_t tr = (sg.time_ray >= 0 && sg.time_ray < rv.n) ? sg.time_ray : b;
+        geom[blockIdx.x] = SegGeom{px, py, tr < rv.n ? rv.t[tr] : 0.0, 0.0};
+    }
+}
+
+// Segment of ray i: the last segment whose begin <= i, if i < its end; -1
+// when no segment holds the ray.  Segments are sorted by begin.
+__device__ __forceinline__ int find_segment(const bt_psf_segment* __restrict__ segs, int nseg,
+                                            int64_t i)
+{
+    int lo = 0, hi = nseg - 1, s = -1;
+    while (lo <= hi) {
+        const int mid = (lo + hi) >> 1;
+        if (segs[mid].begin <= i) { s = mid; lo = mid + 1; } else { hi = mid - 1; }
+    }
+    return (s >= 0 && i < segs[s].end) ? s : -1;
+}
+
+// As k_psf_prep, each ray with its own segment's geometry.
+__global__ __launch_bounds__(kBlock) void k_psf_prep_batch(
+    bt_rays rv, const bt_psf_segment* __restrict__ segs, int nseg,
+    const SegGeom* __restrict__ geom, int keep_a, int keep_b, int ignore_vig,
+    const int* __restrict__ offsets, float* __restrict__ params)
+{
+    __shared__ int s[kBlock / 64];
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const bool v = i < rv.n && psf_valid(rv, i, ignore_vig);
+    const uint64_t m = __ballot(v);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) s[w] = __popcll(m);
+    __syncthreads();
+    if (!v) return;
+    int base = offsets[blockIdx.x];
+    for (int k = 0; k < w; k++) base += s[k];
+    const int rank = base + __popcll(m & ((1ull << lane) - 1ull));
+    float* o = params + (int64_t)rank * kRayStride;
+    const int sg = find_segment(segs, nseg, i);
+    if (sg < 0) {                       // outside every segment: never summed
+        for (int k = 0; k < kRayStride; k++) o[k] = 0.f;
+        return;
+    }
+    const SegGeom g = geom[sg];
+    const bt_psf_segment& d = segs[sg];
+    prep_ray(rv, i, g.p0x, g.p0y, g.t, d.a1[0], d.a1[1], d.a2[0], d.a2[1], keep_a, keep_b, o);
+}
+
+// Number of valid rays before ray index b (b <= n), from the block offsets.
+__device__ __forceinline__ int valid_before(const bt_rays& rv, const int* __restrict__ offsets,
+                                            int64_t b, int ignore_vig)
+{
+    const int64_t blk = b / kBlock;
+    int c = offsets[blk];
+    for (int64_t i = blk * kBlock; i < b; i++) c += psf_valid(rv, i, ignore_vig) ? 1 : 0;
+    return c;
+}
+
+struct SegRange {
+    int begin;      // first compacted ray of the segment
+    int count;      // compacted rays of the segment
+    int items;      // work items (chunks of `chunk` rays)
+    int first;      // index of the segment's first work item
+};
+
+__global__ __launch_bounds__(kBlock) void k_psf_seg_ranges(
+    bt_rays rv, const bt_psf_segment* __restrict__ segs, int nseg, int ignore_vig,
+    const int* __restrict__ offsets, int chunk, SegRange* __restrict__ ranges)
+{
+    const int sg = blockIdx.x * kBlock + threadIdx.x;
+    if (sg >= nseg) return;
+    const bt_psf_segment d = segs[sg];
+    const int64_t b = d.begin < 0 ? 0 : (d.begin > rv.n ? rv.n : d.begin);
+    const int64_t e = d.end < b ? b : (d.end > rv.n ? rv.n : d.end);
+    const int cb = valid_before(rv, offsets, b, ignore_vig);
+    const int ce = valid_before(rv, offsets, e, ignore_vig);
+    SegRange r;
+    r.begin = cb;
+    r.count = ce - cb;
+    r.items = (r.count + chunk - 1) / chunk;
+    r.first = 0;
+    ranges[sg] = r;
+}
+
+struct WorkItem {
+    int seg, r0, r1, pad;
+};
+
+// Exclusive scan of the segments' item counts, then the item list
+// (segment-major, chunks in ray order); items[cap..] are never written and
+// *nitems is clamped to cap.
+__global__ __launch_bounds__(kScanThreads) void k_psf_items(SegRange* __restrict__ ranges,
+                                                            int nseg, int chunk, int cap,
+                                                            WorkItem* __restrict__ items,
+                                                            int* __restrict__ nitems)
+{
+    __shared__ int s[kScanThreads];
+    const int per = (nseg + kScanThreads - 1) / kScanThreads;
+    const int s0 = threadIdx.x * per;
+    int sum = 0;
+    for (int q = s0; q < s0 + per && q < nseg; q++) sum += ranges[q].items;
+    s[threadIdx.x] = sum;
+    __syncthreads();
+    for (int d = 1; d < kScanThreads; d <<= 1) {
+        int v = (threadIdx.x >= d) ? s[threadIdx.x - d] : 0;
+        __syncthreads();
+        s[threadIdx.x] += v;
+        __syncthreads();
+    }
+    int run = s[threadIdx.x] - sum;
+    for (int q = s0; q < s0 + per && q < nseg; q++) {
+        SegRange r = ranges[q];
+        r.first = run;
+        ranges[q] = r;
+        for (int c = 0; c < r.items; c++) {
+            const int it = run + c;
+            if (it >= cap) break;
+            const int r0 = r.begin + c * chunk;
+            const int r1 = min(r.begin + r.count, r0 + chunk);
+            items[it] = WorkItem{q, r0, r1, 0};
+        }
+        run += r.items;
+    }
+    if (threadIdx.x == kScanThreads - 1) {
+        const int total = s[kScanThreads - 1];
+        *nitems = total < cap ? total : cap;
+    }
+}
+
+// Persistent GEMM: workgroup w takes (item, tile) pairs w, w + grid, ... in a
+// fixed assignment, so the result does not depend on scheduling.
+template <int kPrec>
+__global__ __launch_bounds__(256, 2) void k_psf_gemm_batch(
+    const float* __restrict__ params, const WorkItem* __restrict__ items,
+    const int* __restrict__ nitems_ptr, int tiles_i, int tiles, int nx, int ny, int nxp,
+    int nyp, float* __restrict__ partial)
+{
+    const int64_t nwork = (int64_t)(*nitems_ptr) * tiles;
+    for (int64_t wk = blockIdx.x; wk < nwork; wk += gridDim.x) {
+        const int item = (int)(wk / tiles);
+        const int tile = (int)(wk % tiles);
+        const WorkItem it = items[item];
+        psf_tile<kPrec>(params, it.r0, it.r1, (tile % tiles_i) * kTile, (tile / tiles_i) * kTile,
+                        nx, ny, nyp, partial + (int64_t)item * nxp * nyp * 2);
+    }
+}
+
+// amp[s][j][i] = sum over the segment's items of partial[item][i][j], FP64,
+// in item order.  Threads walk j fastest (coalesced float2 reads).
+__global__ __launch_bounds__(kBlock) void k_psf_reduce_batch(
+    const float* __restrict__ partial, const SegRange* __restrict__ ranges, int nseg, int nx,
+    int ny, int nxp, int nyp, int cap, double* __restrict__ amp)
+{
+    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t npix = (int64_t)nx * ny;
+    if (t >= npix * nseg) return;
+    const int sg = (int)(t / npix);
+    const int64_t q = t % npix;
+    const int i = (int)(q / ny), j = (int)(q % ny);
+    const SegRange r = ranges[sg];
+    const int64_t stride = (int64_t)nxp * nyp;
+    const float2* p = reinterpret_cast<const float2*>(partial) + ((int64_t)i * nyp + j);
+    double re = 0.0, im = 0.0;
+    const int last = min(r.first + r.items, cap);
+    for (int k = r.first; k < last; k++) {
+        const float2 v = p[k * stride];
+        re += (double)v.x;
+        im += (double)v.y;
+    }
+    const int64_t pix = (int64_t)sg * npix + (int64_t)j * nx + i;
     amp[2*pix] = re;
     amp[2*pix+1] = im;
 }
@@ -431,4 +668,128 @@ extern "C" int bt_huygens_gemm(bt_rays rays, const double* p0, const double* a1,
     hipLaunchKernelGGL(k_psf_reduce, dim3(grid_for((int64_t)nx * ny)), dim3(kBlock), 0, s,
                        partial, L.ksplit, nx, ny, L.nxp, L.nyp, amp_out);
     return check_launch("bt_huygens_gemm(reduce)");
+}
+
+namespace bt {
+
+struct BatchLayout {
+    int64_t nblocks, cap;
+    int tiles_i, tiles_j, nxp, nyp;
+    int64_t off_geom, off_counts, off_offsets, off_ranges, off_items, off_nitems, off_params,
+        off_partial, bytes;
+};
+
+BatchLayout batch_layout(int64_t n, int nseg, int nx, int ny, int chunk)
+{
+    BatchLayout L;
+    L.nblocks = (n + kBlock - 1) / kBlock;
+    if (L.nblocks < 1) L.nblocks = 1;
+    L.cap = (n + chunk - 1) / chunk + nseg;
+    L.tiles_i = (nx + kTile - 1) / kTile;
+    L.tiles_j = (ny + kTile - 1) / kTile;
+    L.nxp = L.tiles_i * kTile;
+    L.nyp = L.tiles_j * kTile;
+    auto align = [](int64_t v) { return (v + 255) & ~(int64_t)255; };
+    int64_t off = 0;
+    L.off_geom = off;    off = align(off + nseg * (int64_t)sizeof(SegGeom));
+    L.off_counts = off;  off = align(off + L.nblocks * (int64_t)sizeof(int));
+    L.off_offsets = off; off = align(off + (L.nblocks + 1) * (int64_t)sizeof(int));
+    L.off_ranges = off;  off = align(off + nseg * (int64_t)sizeof(SegRange));
+    L.off_items = off;   off = align(off + L.cap * (int64_t)sizeof(WorkItem));
+    L.off_nitems = off;  off = align(off + (int64_t)sizeof(int));
+    L.off_params = off;  off = align(off + (n + kRayGroup) * kRayStride * (int64_t)sizeof(float));
+    L.off_partial = off; off = align(off + L.cap * L.nxp * L.nyp * 2 * (int64_t)sizeof(float));
+    L.bytes = off;
+    return L;
+}
+
+bool batch_args_ok(int64_t n, int nseg, int nx, int ny, int chunk)
+{
+    return n >= 0 && n <= ((int64_t)1 << 30) && nseg >= 1 && nseg <= (1 << 20) && nx >= 1 &&
+           ny >= 1 && nx <= 8192 && ny <= 8192 && chunk >= kRayGroup && chunk <= (1 << 24) &&
+           chunk % kRayGroup == 0 && (int64_t)nx * ny * nseg <= ((int64_t)1 << 31);
+}
+
+}  // namespace bt
+
+extern "C" int64_t bt_huygens_batch_workspace_bytes(int64_t n, int nseg, int nx, int ny,
+                                                    int chunk)
+{
+    if (!batch_args_ok(n, nseg, nx, ny, chunk)) return 0;
+    return batch_layout(n, nseg, nx, ny, chunk).bytes;
+}
+
+extern "C" int bt_huygens_batch(bt_rays rays, const bt_psf_segment* segments, int nseg, int nx,
+                                int ny, int ignore_vignetted, int precision, int chunk,
+                                int nwg, double* amp_out, void* workspace,
+                                int64_t workspace_bytes, void* stream)
+{
+    if (!rays_ok(rays) || !segments || !amp_out || !workspace)
+        return fail(BT_E_NULL, "bt_huygens_batch: null argument");
+    if (!batch_args_ok(rays.n, nseg, nx, ny, chunk))
+        return fail(BT_E_SIZE, "bt_huygens_batch: sizes out of range (rays <= 2^30, "
+                               "1 <= segments <= 2^20, image 1..8192 per side, chunk a "
+                               "multiple of 8)");
+    if (precision != BT_PSF_FP32 && precision != BT_PSF_BF16X3)
+        return fail(BT_E_KIND, "bt_huygens_batch: unknown precision");
+    if (nwg < 0 || nwg > (1 << 20)) return fail(BT_E_SIZE, "bt_huygens_batch: bad workgroup count");
+    const BatchLayout L = batch_layout(rays.n, nseg, nx, ny, chunk);
+    if (workspace_bytes < L.bytes) return fail(BT_E_SIZE, "bt_huygens_batch: workspace too small");
+    char* ws = static_cast<char*>(workspace);
+    SegGeom* geom = reinterpret_cast<SegGeom*>(ws + L.off_geom);
+    int* counts = reinterpret_cast<int*>(ws + L.off_counts);
+    int* offsets = reinterpret_cast<int*>(ws + L.off_offsets);
+    SegRange* ranges = reinterpret_cast<SegRange*>(ws + L.off_ranges);
+    WorkItem* items = reinterpret_cast<WorkItem*>(ws + L.off_items);
+    int* nitems = reinterpret_cast<int*>(ws + L.off_nitems);
+    float* params = reinterpret_cast<float*>(ws + L.off_params);
+    float* partial = reinterpret_cast<float*>(ws + L.off_partial);
+    hipStream_t s = as_stream(stream);
+
+    const int cmax_i = (L.nxp - nx / 2 > nx / 2) ? L.nxp - nx / 2 : nx / 2;
+    const int cmax_j = (L.nyp - ny / 2 > ny / 2) ? L.nyp - ny / 2 : ny / 2;
+    const int keep_a = 24 - bits_for(cmax_i);
+    const int keep_b = 24 - bits_for(cmax_j);
+    const unsigned nb = (unsigned)L.nblocks;
+    const int nbk = rays.n > 0 ? (int)nb : 0;
+
+    hipLaunchKernelGGL(k_psf_seg_geom, dim3(nseg), dim3(kBlock), 0, s, rays, segments, geom);
+    int rc = check_launch("bt_huygens_batch(geometry)");
+    if (rc) return rc;
+    if (rays.n > 0) {
+        hipLaunchKernelGGL(k_psf_count, dim3(nb), dim3(kBlock), 0, s, rays, ignore_vignetted,
+                           counts);
+        rc = check_launch("bt_huygens_batch(count)");
+        if (rc) return rc;
+    }
+    hipLaunchKernelGGL(k_psf_scan, dim3(1), dim3(kScanThreads), 0, s, counts, nbk, offsets);
+    rc = check_launch("bt_huygens_batch(scan)");
+    if (rc) return rc;
+    if (rays.n > 0) {
+        hipLaunchKernelGGL(k_psf_prep_batch, dim3(nb), dim3(kBlock), 0, s, rays, segments, nseg,
+                           geom, keep_a, keep_b, ignore_vignetted, offsets, params);
+        rc = check_launch("bt_huygens_batch(prep)");
+        if (rc) return rc;
+    }
+    hipLaunchKernelGGL(k_psf_seg_ranges, dim3((nseg + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
+                       rays, segments, nseg, ignore_vignetted, offsets, chunk, ranges);
+    rc = check_launch("bt_huygens_batch(ranges)");
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_psf_items, dim3(1), dim3(kScanThreads), 0, s, ranges, nseg, chunk,
+                       (int)L.cap, items, nitems);
+    rc = check_launch("bt_huygens_batch(items)");
+    if (rc) return rc;
+    const int grid = nwg > 0 ? nwg : 512;
+    const int tiles = L.tiles_i * L.tiles_j;
+    if (precision == BT_PSF_FP32)
+        hipLaunchKernelGGL(k_psf_gemm_batch<BT_PSF_FP32>, dim3(grid), dim3(256), 0, s, params,
+                           items, nitems, L.tiles_i, tiles, nx, ny, L.nxp, L.nyp, partial);
+    else
+        hipLaunchKernelGGL(k_psf_gemm_batch<BT_PSF_BF16X3>, dim3(grid), dim3(256), 0, s, params,
+                           items, nitems, L.tiles_i, tiles, nx, ny, L.nxp, L.nyp, partial);
+    rc = check_launch("bt_huygens_batch(mfma)");
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_psf_reduce_batch, dim3(grid_for((int64_t)nx * ny * nseg)), dim3(kBlock),
+                       0, s, partial, ranges, nseg, nx, ny, L.nxp, L.nyp, (int)L.cap, amp_out);
+    return check_launch("bt_huygens_batch(reduce)");
 }

@@ -217,6 +217,77 @@ def bench_newton_start(nrad, naz, reps=5, train_steps=3000):
     return res
 
 
+def bench_batch(reps=5):
+    """C5: the perturbed TMA, 64 fields (default_rng(1)) x 6 wavelengths =
+    384 (field, wavelength) segments with N per segment log-uniform in
+    [1e4, 2e5] (default_rng(2)) as an asGrid pupil of nx = round(sqrt(N)),
+    each traced and turned into a 128 x 128 Huygens PSF at 0.5 um pixels
+    (FP32 MFMA) -- the whole batch as one HIP graph replay.  The work-item
+    size and persistent workgroup count are autotuned on the batch first."""
+    import torch
+    from batoid_amd import workloads
+    from batoid_amd.batch import PSFBatch
+    from batoid_amd.analysis import huygens_amplitudes, _chief_index
+    optic = workloads.perturbed_tma()
+    thx, thy = workloads.tma_fields(n=64, seed=1)
+    rng = np.random.default_rng(2)
+    nseg = 64 * len(workloads.WAVELENGTHS)
+    nrays = np.exp(rng.uniform(np.log(1e4), np.log(2e5), nseg))
+    nx = np.maximum(np.round(np.sqrt(nrays)).astype(int), 2)
+    fx = np.repeat(thx, len(workloads.WAVELENGTHS))
+    fy = np.repeat(thy, len(workloads.WAVELENGTHS))
+    wl = np.tile(workloads.WAVELENGTHS, 64)
+    t0 = time.perf_counter()
+    b = PSFBatch(optic, fx, fy, wl, nx, nxOut=128, dx=0.5e-6, reference='chief',
+                 precision='fp32')
+    torch.cuda.synchronize()
+    log(f"batch: {nseg} segments, {b.n} rays, set up in {time.perf_counter() - t0:.1f} s")
+    timings = b.autotune(reps=2)
+    best = min(timings, key=timings.get)
+    log(f"batch autotune: best chunk {best[0]}, {best[1]} workgroups, {timings[best]:.2f} ms")
+    eager_ms = b.time(reps)
+    b.capture()
+    graph_ms = b.time(reps)
+    nvalid = int(b.valid_rays().sum())
+    nsteps = b.plan.struct.nstep
+    # the trace alone, for the split
+    from batoid_amd.global_vars import _batoid_niter
+    from batoid_amd import _lib, _device
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        _lib.call("bt_trace_plan", b.plan.struct, b.rays_in._rays, b.rays._rays,
+                  _batoid_niter, _device.stream())
+    e1.record()
+    torch.cuda.synchronize()
+    trace_ms = e0.elapsed_time(e1) / reps
+    b.run()
+    amp = b.amplitudes()
+    errs = []
+    for s in (0, nseg // 2, nseg - 1):
+        rays = b.rays[int(b.bounds[s]):int(b.bounds[s + 1])]
+        point = rays.r[_chief_index(int(nx[s]))]
+        ref = huygens_amplitudes(rays, point, b.lattices[s], 128, 128, float(rays.t[0]),
+                                 method='direct')
+        errs.append(float(np.sqrt(np.sum(np.abs(amp[s] - ref) ** 2) / np.sum(np.abs(ref) ** 2))))
+    flop = 8.0 * 128 * 128 * nvalid
+    res = {"workload": "perturbed TMA, 64 fields x 6 wavelengths, uneven asGrid pupils "
+                       "(N log-uniform in [1e4, 2e5]), 128x128 PSF at 0.5 um per segment",
+           "segments": nseg, "rays": b.n, "rays_summed": nvalid, "interfaces": nsteps,
+           "autotuned": {"chunk": best[0], "workgroups": best[1]},
+           "ms_per_batch_graph": graph_ms, "ms_per_batch_eager": eager_ms,
+           "trace_ms": trace_ms, "psf_ms": graph_ms - trace_ms,
+           "rays_per_s": b.n / (graph_ms * 1e-3),
+           "ray_surfaces_per_s": b.n * nsteps / (graph_ms * 1e-3),
+           "psf_latency_ms": graph_ms / nseg,
+           "psf_tflops": flop / ((graph_ms - trace_ms) * 1e-3) / 1e12,
+           "rel_l2_vs_fp64_direct": errs}
+    log(f"batch: {graph_ms:.2f} ms per batch (graph), {eager_ms:.2f} ms eager, trace "
+        f"{trace_ms:.2f} ms, PSF {res['psf_tflops']:.1f} TFLOP/s, rel L2 {max(errs):.1e}")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -228,6 +299,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-huygens", action="store_true")
     ap.add_argument("--no-newton-start", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="also run a plain device copy of this many MiB (calibrates "
                          "FETCH_SIZE / WRITE_SIZE counters against known bytes)")
@@ -351,6 +423,8 @@ def main():
         result["huygens"] = bench_huygens(optic)
     if not args.no_newton_start:
         result["newton_start"] = bench_newton_start(args.nrad, args.naz)
+    if not args.no_batch:
+        result["batch"] = bench_batch()
     if not args.no_cpu:
         log("timing the CPU baseline ...")
         result["cpu_baseline"] = cpu_baseline(optic, batches[0])

@@ -196,8 +196,9 @@ typedef struct bt_plan {
 /* ---- library ----------------------------------------------------------- */
 int bt_abi_version(void);
 const char* bt_last_error(void);
-/* sizeof of bt_rays, bt_surface, bt_medium, bt_obsc, bt_step, bt_plan, in
- * that order, written to sizes[0..n-1]; lets a binding check its layout. */
+/* sizeof of bt_rays, bt_surface, bt_medium, bt_obsc, bt_step, bt_plan,
+ * bt_psf_segment, in that order, written to sizes[0..n-1]; lets a binding
+ * check its layout. */
 int bt_struct_sizes(int64_t* sizes, int n);
 /* Device properties the host uses for launch sizing. */
 int bt_device_info(int* cu_count, int* arch_major, int* arch_minor);
@@ -351,6 +352,33 @@ int bt_huygens_gemm(bt_rays rays, const double* p0, const double* a1,
                     const double* a2, int nx, int ny, double t,
                     int ignore_vignetted, int precision, double* amp_out,
                     void* workspace, int64_t workspace_bytes, void* stream);
+
+/* ---- batched Huygens PSFs ---------------------------------------------------
+ * One nx x ny PSF per segment of a ray batch, all in one call (the nearest
+ * reference is a Python loop of huygensPSF calls, analysis.py:181-305, over
+ * (field, wavelength) pairs; concatenation as in rayVector.py:1351-1365).
+ * Segment s covers rays [begin, end) (segments disjoint, sorted by begin);
+ * its lattice is centred on the position of ray `chief` (chief < 0: the
+ * mean position of its unvignetted rays, analysis.py:289-295) at the time t
+ * of ray `time_ray` (analysis.py:301), with primitive vectors a1, a2.
+ * The segment table lives in device memory, so one call can be captured in a
+ * HIP graph and replayed on new ray data: no host synchronisation, no
+ * allocation, no memset.  Rays are summed in work items of `chunk` rays (a
+ * multiple of 8) by `nwg` persistent workgroups (0: 512) in a fixed
+ * assignment; items of one segment are added in FP64 in item order.
+ * `amp_out` receives [nseg][ny][nx][2] complex128.                         */
+typedef struct bt_psf_segment {
+    int64_t begin, end;
+    int64_t chief;
+    int64_t time_ray;
+    double a1[2], a2[2];
+} bt_psf_segment;
+
+int64_t bt_huygens_batch_workspace_bytes(int64_t n, int nseg, int nx, int ny, int chunk);
+int bt_huygens_batch(bt_rays rays, const bt_psf_segment* segments, int nseg, int nx,
+                     int ny, int ignore_vignetted, int precision, int chunk, int nwg,
+                     double* amp_out, void* workspace, int64_t workspace_bytes,
+                     void* stream);
 
 #ifdef __cplusplus
 }

@@ -39,7 +39,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_struct_layouts_match():
     lib = _lib.load()
-    sizes = (ctypes.c_int64 * 6)()
-    assert lib.bt_struct_sizes(sizes, 6) == 0
+    sizes = (ctypes.c_int64 * len(_lib.STRUCTS))()
+    assert lib.bt_struct_sizes(sizes, len(_lib.STRUCTS)) == 0
     assert list(sizes) == [ctypes.sizeof(s) for s in _lib.STRUCTS]
     assert lib.bt_abi_version() == 2
