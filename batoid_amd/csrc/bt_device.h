@@ -329,8 +329,17 @@ BT_DEV bool same_bits(double a, double b)
 // neighbouring doubles dt'' == dt -- every remaining iterate is known, and the
 // loop stops with exactly the state (dt, rP, sag) that running all `niter`
 // iterations would leave.  Results are bit-identical to the full loop.
+// The last surface normal Newton evaluated and the dt of the point it was
+// evaluated at: a reflection or refraction at that same point reuses it
+// (surf_normal of bit-identical arguments is bit-identical).
+struct NormalMemo {
+    double dt, nx, ny, nz;
+    bool ok;
+};
+
 BT_DEV bool newton(const bt_surface* s, double x, double y, double z,
-                   double vx, double vy, double vz, double& dt, int niter)
+                   double vx, double vy, double vz, double& dt, int niter,
+                   NormalMemo* memo = nullptr)
 {
     double rPx = x+vx*dt;
     double rPy = y+vy*dt;
@@ -340,6 +349,7 @@ BT_DEV bool newton(const bt_surface* s, double x, double y, double z,
     for (int iter = 0; iter < niter; iter++) {
         double nx, ny, nz;
         surf_normal(s, rPx, rPy, nx, ny, nz);
+        if (memo) *memo = NormalMemo{dt, nx, ny, nz, true};
         double dtn = (rPx-x)*nx + (rPy-y)*ny + (sz-z)*nz;
         dtn /= (nx*vx + ny*vy + nz*vz);
         if (same_bits(dtn, dt)) break;            // fixed point: state unchanged
@@ -392,7 +402,8 @@ BT_DEV bool quadric_tti(const bt_surface* s, double x, double y, double z,
 }
 
 BT_DEV bool leaf_tti(const bt_surface* s, double x, double y, double z,
-                     double vx, double vy, double vz, double& dt, int niter)
+                     double vx, double vy, double vz, double& dt, int niter,
+                     NormalMemo* memo = nullptr)
 {
     switch (s->kind) {
     case BT_PLANE:                                   // plane.cpp:25-31
@@ -443,7 +454,7 @@ BT_DEV bool leaf_tti(const bt_surface* s, double x, double y, double z,
         return quadric_tti(s, x, y, z, vx, vy, vz, dt);
     case BT_ASPHERE:                                 // asphere.cpp:75-85
         if (!quadric_tti(s, x, y, z, vx, vy, vz, dt)) return false;
-        return newton(s, x, y, z, vx, vy, vz, dt, niter);
+        return newton(s, x, y, z, vx, vy, vz, dt, niter, memo);
     case BT_TILTED: {                                // tilted.cpp:29-37
         double num = z - x*s->p[0] - y*s->p[1];
         double den = vx*s->p[0] + vy*s->p[1] - vz;
@@ -453,18 +464,19 @@ BT_DEV bool leaf_tti(const bt_surface* s, double x, double y, double z,
     }
     case BT_BICUBIC:
     case BT_POLYNOMIAL:                              // surface.cpp:22-49
-        return newton(s, x, y, z, vx, vy, vz, dt, niter);
+        return newton(s, x, y, z, vx, vy, vz, dt, niter, memo);
     }
     return false;
 }
 
 BT_DEV bool surf_tti(const bt_surface* s, double x, double y, double z,
-                     double vx, double vy, double vz, double& dt, int niter)
+                     double vx, double vy, double vz, double& dt, int niter,
+                     NormalMemo* memo = nullptr)
 {
-    if (s->kind != BT_SUM) return leaf_tti(s, x, y, z, vx, vy, vz, dt, niter);
+    if (s->kind != BT_SUM) return leaf_tti(s, x, y, z, vx, vy, vz, dt, niter, memo);
     // sum.cpp:47-56: first child's intersection seeds Newton on the Sum
     if (!leaf_tti(s + 1, x, y, z, vx, vy, vz, dt, niter)) return false;
-    return newton(s, x, y, z, vx, vy, vz, dt, niter);
+    return newton(s, x, y, z, vx, vy, vz, dt, niter, memo);
 }
 
 // The surface's analytic base start: the root of a Sum's first child or of an
@@ -704,9 +716,20 @@ BT_DEV bool apply_interaction(int action, const double* __restrict__ dr,
     double vz = r.vx*drot[2] + r.vy*drot[5] + r.vz*drot[8];
     double t = r.t;
     double dt = 0.0;
+    NormalMemo memo;
+    memo.ok = false;
     const bool hit = (start == BT_START_REFERENCE)
-                         ? surf_tti(surf, x, y, z, vx, vy, vz, dt, niter)
+                         ? surf_tti(surf, x, y, z, vx, vy, vz, dt, niter, &memo)
                          : fast_tti(surf, x, y, z, vx, vy, vz, dt, niter, start, seed);
+    // the normal at the intersection, reusing Newton's when it was evaluated
+    // at this very point
+    auto normal_at = [&](double px, double py, double& nx, double& ny, double& nz) {
+        if (memo.ok && same_bits(memo.dt, dt)) {
+            nx = memo.nx; ny = memo.ny; nz = memo.nz;
+        } else {
+            surf_normal(surf, px, py, nx, ny, nz);
+        }
+    };
     if (!hit) {
         r.fail = true;
         r.vig = true;
@@ -718,7 +741,7 @@ BT_DEV bool apply_interaction(int action, const double* __restrict__ dr,
     t += dt;
     if (action == BT_ACT_REFLECT) {                  // batoid.cpp:340-353
         double nx, ny, nz;
-        surf_normal(surf, x, y, nx, ny, nz);
+        normal_at(x, y, nx, ny, nz);
         double alpha = vx*nx;
         alpha += vy*ny;
         alpha += vz*nz;
@@ -734,7 +757,7 @@ BT_DEV bool apply_interaction(int action, const double* __restrict__ dr,
         double nvy = vy*n1;
         double nvz = vz*n1;
         double nx, ny, nz;
-        surf_normal(surf, x, y, nx, ny, nz);
+        normal_at(x, y, nx, ny, nz);
         double alpha = nvx*nx;
         alpha += nvy*ny;
         alpha += nvz*nz;

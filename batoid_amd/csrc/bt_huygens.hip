@@ -201,11 +201,13 @@ __device__ __forceinline__ void psf_tile(const float* __restrict__ params, int r
 
     if (kPrec == BT_PSF_FP32) {
         // 32x32x2: lane supplies A'[l32][half], B'[half][l32]; one ray per MFMA
-        //   (prow, half) = (0,0): Re U = f cos;  (0,1): -Im U = -f sin
-        //                  (1,0): Im U = f sin;  (1,1): Re U = f cos
-        const float sa = (prow == 0 && half == 1) ? -1.f : 1.f;
-        const float oa = (prow == half) ? 0.25f : 0.f;
-        const float ob = (half == 0) ? 0.25f : 0.f;    // Re V = cos, Im V = sin
+        //   (prow, half) = (0,0): Re U = cos = sin(th + 1/4)
+        //                  (0,1): -Im U = -sin = sin(th + 1/2)
+        //                  (1,0): Im U = sin(th);  (1,1): Re U = sin(th + 1/4)
+        // (th in revolutions), so every A value is one sine with a per-lane
+        // offset; the ray's flux scales its B values (Re V = cos, Im V = sin).
+        const float oa = (prow == half) ? 0.25f : (half == 1 ? 0.5f : 0.f);
+        const float ob = (half == 0) ? 0.25f : 0.f;
         // rays in groups of 4 whose parameters (wave-uniform) are fetched one
         // group ahead of their use
         constexpr int G = 4;
@@ -224,12 +226,12 @@ __device__ __forceinline__ void psf_tile(const float* __restrict__ params, int r
             if (r + G < r1) fetch(nxt, r + G);
             for (int q = 0; q < G; q++) {
                 const float ah = cur[q][0], al = cur[q][1], bh = cur[q][2], bl = cur[q][3];
-                const float ph0 = cur[q][4], f = cur[q][5];
+                const float pa = cur[q][4] + oa, f = cur[q][5];
                 float av[4], bv[2];
                 for (int rb = 0; rb < 4; rb++)
-                    av[rb] = f * sin_rev(__builtin_fmaf(sa, step_phase(ci[rb], ah, al, ph0), oa));
+                    av[rb] = sin_rev(step_phase(ci[rb], ah, al, pa));
                 for (int cb = 0; cb < 2; cb++)
-                    bv[cb] = sin_rev(step_phase(cj[cb], bh, bl, ob));
+                    bv[cb] = f * sin_rev(step_phase(cj[cb], bh, bl, ob));
                 for (int rb = 0; rb < 4; rb++)
                     for (int cb = 0; cb < 2; cb++)
                         acc[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[rb], bv[cb],
@@ -311,26 +313,45 @@ __global__ __launch_bounds__(256, 2) void k_psf_gemm(
                     nyp, partial + (int64_t)ks * nxp * nyp * 2);
 }
 
-// amp[j][i] = sum_ks partial[ks][i][j] in FP64, fixed order.  Threads walk
-// j fastest so the partial-image reads are coalesced float2 loads.
+// amp[j][i] = sum_ks partial[ks][i][j] in FP64, in a fixed order: a block
+// covers 32 consecutive pixels (j fastest, coalesced float2 reads) with 8
+// lanes per pixel, lane g summing splits g, g+8, ...; the 8 sums are then
+// added in g order.
+constexpr int kRedPix = 32;
+constexpr int kRedLanes = kBlock / kRedPix;
+
+__device__ __forceinline__ void reduce_splits(const float2* __restrict__ p, int64_t stride,
+                                              int k0, int k1, bool live, double* __restrict__ out)
+{
+    __shared__ double s_re[kRedLanes][kRedPix], s_im[kRedLanes][kRedPix];
+    const int pj = threadIdx.x % kRedPix, g = threadIdx.x / kRedPix;
+    double re = 0.0, im = 0.0;
+    if (live)
+        for (int k = k0 + g; k < k1; k += kRedLanes) {
+            const float2 v = p[k * stride];
+            re += (double)v.x;
+            im += (double)v.y;
+        }
+    s_re[g][pj] = re;
+    s_im[g][pj] = im;
+    __syncthreads();
+    if (g == 0 && live) {
+        double r = 0.0, m = 0.0;
+        for (int q = 0; q < kRedLanes; q++) { r += s_re[q][pj]; m += s_im[q][pj]; }
+        out[0] = r;
+        out[1] = m;
+    }
+}
+
 __global__ __launch_bounds__(kBlock) void k_psf_reduce(const float* __restrict__ partial,
                                                        int ksplit, int nx, int ny, int nxp,
                                                        int nyp, double* __restrict__ amp)
 {
-    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (t >= (int64_t)nx * ny) return;
-    const int i = (int)(t / ny), j = (int)(t % ny);
-    const int64_t stride = (int64_t)nxp * nyp;
+    const int64_t q = (int64_t)blockIdx.x * kRedPix + threadIdx.x % kRedPix;
+    const bool live = q < (int64_t)nx * ny;
+    const int i = live ? (int)(q / ny) : 0, j = live ? (int)(q % ny) : 0;
     const float2* p = reinterpret_cast<const float2*>(partial) + ((int64_t)i * nyp + j);
-    double re = 0.0, im = 0.0;
-    for (int k = 0; k < ksplit; k++) {
-        const float2 v = p[k * stride];
-        re += (double)v.x;
-        im += (double)v.y;
-    }
-    const int64_t pix = (int64_t)j * nx + i;
-    amp[2*pix] = re;
-    amp[2*pix+1] = im;
+    reduce_splits(p, (int64_t)nxp * nyp, 0, ksplit, live, amp + 2 * ((int64_t)j * nx + i));
 }
 
 // ---------------------------------------------------------------------------
@@ -526,30 +547,22 @@ __global__ __launch_bounds__(256, 2) void k_psf_gemm_batch(
 }
 
 // amp[s][j][i] = sum over the segment's items of partial[item][i][j], FP64,
-// in item order.  Threads walk j fastest (coalesced float2 reads).
+// in the fixed order of reduce_splits.
 __global__ __launch_bounds__(kBlock) void k_psf_reduce_batch(
     const float* __restrict__ partial, const SegRange* __restrict__ ranges, int nseg, int nx,
     int ny, int nxp, int nyp, int cap, double* __restrict__ amp)
 {
-    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     const int64_t npix = (int64_t)nx * ny;
-    if (t >= npix * nseg) return;
-    const int sg = (int)(t / npix);
-    const int64_t q = t % npix;
-    const int i = (int)(q / ny), j = (int)(q % ny);
+    const int64_t blocks_per_seg = (npix + kRedPix - 1) / kRedPix;
+    const int sg = (int)(blockIdx.x / blocks_per_seg);
+    const int64_t q = (int64_t)(blockIdx.x % blocks_per_seg) * kRedPix + threadIdx.x % kRedPix;
+    const bool live = q < npix;
+    const int i = live ? (int)(q / ny) : 0, j = live ? (int)(q % ny) : 0;
     const SegRange r = ranges[sg];
-    const int64_t stride = (int64_t)nxp * nyp;
-    const float2* p = reinterpret_cast<const float2*>(partial) + ((int64_t)i * nyp + j);
-    double re = 0.0, im = 0.0;
     const int last = min(r.first + r.items, cap);
-    for (int k = r.first; k < last; k++) {
-        const float2 v = p[k * stride];
-        re += (double)v.x;
-        im += (double)v.y;
-    }
-    const int64_t pix = (int64_t)sg * npix + (int64_t)j * nx + i;
-    amp[2*pix] = re;
-    amp[2*pix+1] = im;
+    const float2* p = reinterpret_cast<const float2*>(partial) + ((int64_t)i * nyp + j);
+    reduce_splits(p, (int64_t)nxp * nyp, r.first, last, live,
+                  amp + 2 * ((int64_t)sg * npix + (int64_t)j * nx + i));
 }
 
 struct GemmLayout {
@@ -665,8 +678,8 @@ extern "C" int bt_huygens_gemm(bt_rays rays, const double* p0, const double* a1,
                            L.tiles_i, L.ksplit, L.nxp, L.nyp, partial);
     rc = check_launch("bt_huygens_gemm(mfma)");
     if (rc) return rc;
-    hipLaunchKernelGGL(k_psf_reduce, dim3(grid_for((int64_t)nx * ny)), dim3(kBlock), 0, s,
-                       partial, L.ksplit, nx, ny, L.nxp, L.nyp, amp_out);
+    hipLaunchKernelGGL(k_psf_reduce, dim3((unsigned)(((int64_t)nx * ny + kRedPix - 1) / kRedPix)),
+                       dim3(kBlock), 0, s, partial, L.ksplit, nx, ny, L.nxp, L.nyp, amp_out);
     return check_launch("bt_huygens_gemm(reduce)");
 }
 
@@ -789,7 +802,8 @@ extern "C" int bt_huygens_batch(bt_rays rays, const bt_psf_segment* segments, in
                            items, nitems, L.tiles_i, tiles, nx, ny, L.nxp, L.nyp, partial);
     rc = check_launch("bt_huygens_batch(mfma)");
     if (rc) return rc;
-    hipLaunchKernelGGL(k_psf_reduce_batch, dim3(grid_for((int64_t)nx * ny * nseg)), dim3(kBlock),
-                       0, s, partial, ranges, nseg, nx, ny, L.nxp, L.nyp, (int)L.cap, amp_out);
+    const int64_t red_blocks = (((int64_t)nx * ny + kRedPix - 1) / kRedPix) * nseg;
+    hipLaunchKernelGGL(k_psf_reduce_batch, dim3((unsigned)red_blocks), dim3(kBlock), 0, s, partial,
+                       ranges, nseg, nx, ny, L.nxp, L.nyp, (int)L.cap, amp_out);
     return check_launch("bt_huygens_batch(reduce)");
 }

@@ -30,6 +30,14 @@ BF16_MFMA_PEAK_TFLOPS = 2500.0
 # Bytes the fused trace moves per ray: reads x, y, z, vx, vy, vz, t,
 # wavelength (8 B each) + 2 flag bytes; writes x..t + 2 flag bytes.
 BYTES_PER_RAY = 8 * 8 + 2 + 7 * 8 + 2
+# HBM bytes per ray of the same kernel on this workload, measured with
+# rocprofv3 FETCH_SIZE and WRITE_SIZE (each scaled by its reading for a
+# 1 GiB device copy: x2.000 and x1.000), from
+# profiles/trace_plan_tma_1e7_counters.md.
+MEASURED_TRAFFIC_PER_RAY = 124.04
+# FP64 flops per ray of the same kernel on this workload: 64 lanes x
+# (ADD + MUL + 2 FMA) FP64 wave instructions (SQ_INSTS_VALU_*_F64), same file.
+FP64_FLOP_PER_RAY = 8271.0
 
 
 def log(*a):
@@ -414,9 +422,18 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": MEASURED_TRAFFIC_PER_RAY * n,
             "bytes_per_launch": bytes_per_launch,
             "kernel_ms": kernel_ms,
+            # the kernel keeps rays in registers across all 12 interfaces and
+            # is FP64-VALU bound; its compute roofline:
+            "fp64": {
+                "achieved": FP64_FLOP_PER_RAY * n / (kernel_ms * 1e-3) / 1e12,
+                "peak": FP64_PEAK_TFLOPS,
+                "unit": "TFLOP/s",
+                "frac": FP64_FLOP_PER_RAY * n / (kernel_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
+                "flop_per_ray": FP64_FLOP_PER_RAY,
+            },
         },
     }
     if not args.no_huygens:
