@@ -106,7 +106,9 @@ class NewtonStartNet:
             seed=0, log_every=250):
         """Train on ``features`` [n, 6] (float32, device) against ``delta``
         [n] (dt* - dt_conic, float64 metres).  Adam with a cosine learning
-        rate from ``lr`` to ``lr_final``; minibatches drawn with replacement."""
+        rate from ``lr`` to ``lr_final``; minibatches drawn with replacement.
+        Returns the (step, minibatch loss) history, logged at step 1 and every
+        ``log_every`` steps (the loss is the one before that step's update)."""
         dev = _device.device()
         features = features.to(dev, torch.float32).contiguous()
         delta = delta.to(dev, torch.float64)
@@ -137,7 +139,7 @@ class NewtonStartNet:
             rate = lr_final + 0.5 * (lr - lr_final) * (1 + math.cos(math.pi * frac))
             _lib.call("bt_mlp_adam", _fptr(self.params), _fptr(grad), _fptr(m), _fptr(v),
                       self.params.numel(), rate, 0.9, 0.999, 1e-8, step, st)
-            if log_every and (step % log_every == 0 or step == steps):
+            if log_every and (step == 1 or step % log_every == 0 or step == steps):
                 self.history.append((step, float(loss.item())))
         self._packed = None
         return self.history

@@ -135,3 +135,52 @@ def perturbed_tma(jmax=28, zernike_rms=50e-9, zernike_seed=577215, grid=204,
         surf = Sum(iface.surface, Zernike(coef, R_outer=R), Bicubic(xs, xs, noise))
         optic = optic.withSurface(name, surf)
     return optic
+
+
+def _asphere_sag(asph, x, y):
+    """Host sag of an Asphere (quadric + sum_i alpha_i r^(2i)), for fitting."""
+    r2 = x * x + y * y
+    R, k = asph.R, asph.conic
+    out = r2 / (R * (1 + np.sqrt(1 - (1 + k) * r2 / R ** 2)))
+    for i, a in enumerate(asph._padded_coefs):
+        out = out + a * r2 ** (i + 1)
+    return out
+
+
+def _fit_zernike(fun, R_outer, R_inner, jmax, npts=20000, seed=1):
+    """Least-squares Zernike coefficients (Noll 1..jmax, circular basis over
+    R_outer) of fun(x, y) sampled uniformly over the annulus."""
+    from .zernike import zernike_coef_arrays
+    rng = np.random.default_rng(seed)
+    r = np.sqrt(rng.uniform(R_inner ** 2, R_outer ** 2, npts))
+    th = rng.uniform(0, 2 * np.pi, npts)
+    x, y = r * np.cos(th), r * np.sin(th)
+    cols = []
+    for j in range(1, jmax + 1):
+        e = np.zeros(jmax + 1)
+        e[j] = 1.0
+        c, _, _ = zernike_coef_arrays(e, R_outer, 0.0)
+        cols.append(np.polynomial.polynomial.polyval2d(x, y, c))
+    A = np.stack(cols, axis=1)
+    sol, *_ = np.linalg.lstsq(A, fun(x, y), rcond=None)
+    return np.concatenate([[0.0], sol])
+
+
+def freeform_tma(jfit=45, **kwargs):
+    """The perturbed TMA with freeform mirrors: each mirror is
+    Sum(Zernike(fit of its asphere + the 50 nm perturbation, j <= 45),
+    Bicubic) -- the same optical surfaces to ~1e-8 m, but with no conic term,
+    so the first child's Newton starts from the ray origin (dt = 0) as the
+    reference does for polynomial surfaces (surface.cpp:22-49)."""
+    from .surface import Sum, Zernike
+    base = perturbed_tma(**kwargs)
+    optic = base
+    for name in ("M1", "M2", "M3"):
+        iface = base[name]
+        asph, zern, bic = iface.surface.surfaces
+        R = iface.R_outer
+        fit = _fit_zernike(lambda x, y: _asphere_sag(asph, x, y), R, iface.R_inner, jfit)
+        coef = fit.copy()
+        coef[:len(zern.coef)] += zern.coef
+        optic = optic.withSurface(name, Sum(Zernike(coef, R_outer=R), bic))
+    return optic

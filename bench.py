@@ -159,16 +159,19 @@ def bench_huygens(optic, reps=10):
     return res
 
 
-def bench_newton_start(nrad, naz, reps=5, train_steps=3000):
-    """The perturbed TMA (every mirror Sum(Asphere, Zernike j<=28, Bicubic
-    204^2)) traced with the reference start, the conic start and learned
-    starts.  The networks are trained first on 12 other fields (recorded
-    with their wall time); each mode is then timed on one 1e7-ray batch and
-    checked against the reference-start trace (max |dx|, flags)."""
+def bench_newton_start(nrad, naz, optic_name="perturbed_tma", reps=5, train_steps=3000):
+    """An optic with Newton mirrors traced with the reference start, the
+    analytic base start ('conic': the mirror's conic root, or the ray origin
+    for polynomial mirrors) and learned starts.  perturbed_tma: every mirror
+    Sum(Asphere, Zernike j<=28, Bicubic 204^2); freeform_tma: the same
+    mirrors as Sum(Zernike fit, Bicubic), no conic base.  The networks are
+    trained first on 12 other fields (recorded with their wall time); each
+    mode is then timed on one 1e7-ray batch and checked against the
+    reference-start trace (max |dx|, flags)."""
     import torch
     from batoid_amd import workloads
     from batoid_amd.newton_start import train_newton_starts
-    optic = workloads.perturbed_tma()
+    optic = getattr(workloads, optic_name)()
     thx, thy = workloads.tma_fields()
     t0 = time.perf_counter()
     train_rvs = [workloads.tma_rays(optic, thx[k], thy[k], 620e-9, nrad=60, naz=600)
@@ -181,8 +184,11 @@ def bench_newton_start(nrad, naz, reps=5, train_steps=3000):
     rv0 = workloads.tma_rays(optic, thx[0], thy[0], 620e-9, nrad=nrad, naz=naz)
     n = len(rv0)
     nsteps = optic._plan(rv0.coordSys, False).struct.nstep
-    res = {"workload": "perturbed TMA (Sum(Asphere, Zernike j<=28 N(0,50nm), "
-                       "Bicubic 204^2 10nm rms) per mirror), field 0, 620 nm",
+    res = {"workload": {"perturbed_tma": "perturbed TMA (Sum(Asphere, Zernike j<=28 "
+                                          "N(0,50nm), Bicubic 204^2 10nm rms) per mirror)",
+                        "freeform_tma": "freeform TMA (Sum(Zernike j<=45 fit of the asphere "
+                                         "+ perturbation, Bicubic) per mirror)"}[optic_name]
+                       + ", field 0, 620 nm",
            "rays": n, "interfaces": nsteps, "train_seconds": train_s,
            "train_samples_per_surface": 12 * len(train_rvs[0]),
            "train_steps": train_steps}
@@ -439,7 +445,9 @@ def main():
     if not args.no_huygens:
         result["huygens"] = bench_huygens(optic)
     if not args.no_newton_start:
-        result["newton_start"] = bench_newton_start(args.nrad, args.naz)
+        result["newton_start"] = {
+            name: bench_newton_start(args.nrad, args.naz, name)
+            for name in ("perturbed_tma", "freeform_tma")}
     if not args.no_batch:
         result["batch"] = bench_batch()
     if not args.no_cpu:

@@ -134,9 +134,15 @@ def test_samples_match_reference_intersection():
     np.testing.assert_allclose(f[ok, 5], loc.vz[ok], rtol=1e-7)
 
 
-@pytest.fixture(scope="module")
-def trained():
-    optic = workloads.perturbed_tma()
+OPTICS = {"perturbed": "perturbed_tma", "freeform": "freeform_tma"}
+
+
+@pytest.fixture(scope="module", params=sorted(OPTICS))
+def trained(request):
+    """Networks for every Newton surface of the perturbed TMA (conic base
+    start) and of its freeform variant (polynomial mirrors: plain start)."""
+    print(f"case train {request.param}", flush=True)
+    optic = getattr(workloads, OPTICS[request.param])()
     thx, thy = workloads.tma_fields()
     rvs = [workloads.tma_rays(optic, thx[k], thy[k], 620e-9, nrad=40, naz=400)
            for k in FIELDS_TRAIN]
