@@ -16,7 +16,7 @@ own forward/backward and Adam kernels, and inference inside the fused trace
 is an fp16 MFMA product per wave.
 
 The start changes only where Newton begins: every ray is still converged on
-the full surface (to a sag gap of 1e-13 m) and any ray that does not
+the full surface (to a sag gap of 1e-14 m) and any ray that does not
 converge from the learned start is re-intersected on the reference path.
 
 Usage::

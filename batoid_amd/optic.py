@@ -269,7 +269,7 @@ class CompoundOptic(Optic):
 
         ``start`` picks how Newton surfaces seed their intersection:
         "reference" (default; bit-identical to the reference), "conic" (the
-        base conic's root, converged to a 1e-13 m sag gap), or a mapping
+        base conic's root, converged to a 1e-14 m sag gap), or a mapping
         {interface name: NewtonStartNet} (learned starts, see
         newton_start.py).  Fast starts fall back to the reference path for
         any ray they do not converge."""

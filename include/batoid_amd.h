@@ -158,7 +158,7 @@ enum bt_action { BT_ACT_INTERSECT = 0, BT_ACT_REFLECT = 1, BT_ACT_REFRACT = 2 };
  *              bit-identical to the reference;
  *   CONIC      the analytic root of the base conic (Asphere's quadric, or a
  *              Sum's first child), then Newton on the full surface until the
- *              sag gap is <= 1e-13 m (at most 8 iterations);
+ *              sag gap is <= 1e-14 m (at most 8 iterations);
  *   MLP        as CONIC, plus the learned correction of the step's network.
  * A ray that does not converge from a fast start is re-intersected on the
  * REFERENCE path, so failure semantics stay the reference's.  Starts other

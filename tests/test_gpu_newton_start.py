@@ -199,7 +199,7 @@ def test_fast_start_trace_matches_reference(trained, mode):
                 assert np.abs(a[c][ok] - b[c][ok]).max() <= 1e-14, (mode, k, wl, c)
 
 
-def test_fast_start_unperturbed_and_ragged(trained):
+def test_fast_start_ragged_sizes_and_partial_networks(trained):
     """Ragged sizes (lanes past the end in the last wave) and a plan whose
     networks cover only some surfaces (the others run the conic start)."""
     optic, nets = trained
