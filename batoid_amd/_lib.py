@@ -88,6 +88,7 @@ ACT_INTERSECT, ACT_REFLECT, ACT_REFRACT = range(3)
 START_REFERENCE, START_CONIC, START_MLP = range(3)
 PLAN_NO_REFRACT = 1
 PLAN_FAST_START = 2
+PLAN_MLP_START = 4
 PSF_FP32, PSF_BF16X3 = 0, 1
 OBSC_MAX_STACK = 32
 MAX_SUM_CHILDREN = 8

@@ -135,6 +135,51 @@ BT_DEV double horner2d(double x, double y, const double* __restrict__ c,
     return r;
 }
 
+// table_eval and table_grad at one point, sharing the cell and its 16 table
+// values; each result is bit-identical to the separate function's.
+BT_DEV void table_eval_grad(const bt_surface* s, double x, double y, double& val,
+                            double& dzdx, double& dzdy)
+{
+    int ix, iy;
+    if (!table_cell(s, x, y, ix, iy)) {
+        double v = s->flags ? __builtin_nan("") : 0.0;
+        val = v; dzdx = v; dzdy = v;
+        return;
+    }
+    const double dx = s->p[2], dy = s->p[3];
+    const double* __restrict__ z = s->a0;
+    const double* __restrict__ zx = s->a1;
+    const double* __restrict__ zy = s->a2;
+    const double* __restrict__ zxy = s->a3;
+    double xgrid = s->p[0] + ix*dx;
+    double ygrid = s->p[1] + iy*dy;
+    double xfrac = (x - xgrid)/dx;
+    double yfrac = (y - ygrid)/dy;
+    const int64_t k0 = (int64_t)s->n1*iy + ix, k1 = k0 + s->n1;
+    const double z00 = z[k0], z01 = z[k0+1], z10 = z[k1], z11 = z[k1+1];
+    const double x00 = zx[k0], x01 = zx[k0+1], x10 = zx[k1], x11 = zx[k1+1];
+    const double y00 = zy[k0], y01 = zy[k0+1], y10 = zy[k1], y11 = zy[k1+1];
+    const double c00 = zxy[k0], c01 = zxy[k0+1], c10 = zxy[k1], c11 = zxy[k1+1];
+
+    double val0 = spline1(xfrac, z00, z01, x00*dx, x01*dx);
+    double val1 = spline1(xfrac, z10, z11, x10*dx, x11*dx);
+    double der0 = spline1(xfrac, y00, y01, c00*dx, c01*dx);
+    double der1 = spline1(xfrac, y10, y11, c10*dx, c11*dx);
+    val = spline1(yfrac, val0, val1, der0*dy, der1*dy);
+
+    val0 = grad1(xfrac, z00, z01, x00*dx, x01*dx);
+    val1 = grad1(xfrac, z10, z11, x10*dx, x11*dx);
+    der0 = grad1(xfrac, y00, y01, c00*dx, c01*dx);
+    der1 = grad1(xfrac, y10, y11, c10*dx, c11*dx);
+    dzdx = spline1(yfrac, val0, val1, der0*dy, der1*dy)/dx;
+
+    val0 = grad1(yfrac, z00, z10, y00*dy, y10*dy);
+    val1 = grad1(yfrac, z01, z11, y01*dy, y11*dy);
+    der0 = grad1(yfrac, x00, x10, c00*dy, c10*dy);
+    der1 = grad1(yfrac, x01, x11, c01*dy, c11*dy);
+    dzdy = spline1(xfrac, val0, val1, der0*dx, der1*dx)/dy;
+}
+
 // ---------------------------------------------------------------------------
 // Quadric family (src/quadric.cpp, src/asphere.cpp)
 // ---------------------------------------------------------------------------
@@ -311,6 +356,46 @@ BT_DEV void surf_normal(const bt_surface* s, double x, double y,
         sx += tnx/tnz;
         sy += tny/tnz;
     }
+    nz = 1./sqrt(sx*sx + sy*sy + 1);
+    nx = sx*nz;
+    ny = sy*nz;
+}
+
+// Sag and normal at one point (the reference evaluates them separately; the
+// results here are bit-identical, with shared work evaluated once).
+BT_DEV void leaf_sag_normal(const bt_surface* s, double x, double y, double& sag,
+                            double& nx, double& ny, double& nz)
+{
+    if (s->kind == BT_BICUBIC) {                     // bicubic.cpp:19-40
+        double dxdz, dydz;
+        table_eval_grad(s, x, y, sag, dxdz, dydz);
+        if (__builtin_isnan(dxdz)) {
+            nx = __builtin_nan(""); ny = nx; nz = nx;
+            return;
+        }
+        nz = 1/sqrt(1 + dxdz*dxdz + dydz*dydz);
+        nx = -dxdz*nz;
+        ny = -dydz*nz;
+        return;
+    }
+    sag = leaf_sag(s, x, y);
+    leaf_normal(s, x, y, nx, ny, nz);
+}
+
+BT_DEV void surf_sag_normal(const bt_surface* s, double x, double y, double& sag,
+                            double& nx, double& ny, double& nz)
+{
+    if (s->kind != BT_SUM) { leaf_sag_normal(s, x, y, sag, nx, ny, nz); return; }
+    double result = 0.0, sx = 0.0, sy = 0.0;
+    const int nc = s->nchild;
+    for (int i = 0; i < nc; i++) {
+        double ts, tnx, tny, tnz;
+        leaf_sag_normal(s + 1 + i, x, y, ts, tnx, tny, tnz);
+        result += ts;
+        sx += tnx/tnz;
+        sy += tny/tnz;
+    }
+    sag = result;
     nz = 1./sqrt(sx*sx + sy*sy + 1);
     nx = sx*nz;
     ny = sy*nz;
@@ -516,42 +601,44 @@ constexpr int kFastIter = 8;
 // optic, reach ~1e-14 in the final direction cosines; 1e-14 m keeps them an
 // order of magnitude below that.
 BT_DEV bool newton_fast(const bt_surface* s, double x, double y, double z,
-                        double vx, double vy, double vz, double& dt)
+                        double vx, double vy, double vz, double& dt, NormalMemo* memo)
 {
     double rPx = x+vx*dt, rPy = y+vy*dt, rPz = z+vz*dt;
-    double sz = surf_sag(s, rPx, rPy);
+    double sz, nx, ny, nz;
+    surf_sag_normal(s, rPx, rPy, sz, nx, ny, nz);
     for (int it = 0; it < kFastIter; it++) {
-        if (fabs(sz-rPz) <= kFastTol) return true;
-        double nx, ny, nz;
-        surf_normal(s, rPx, rPy, nx, ny, nz);
+        if (fabs(sz-rPz) <= kFastTol) break;
         double d = (rPx-x)*nx + (rPy-y)*ny + (sz-z)*nz;
         dt = d / (nx*vx + ny*vy + nz*vz);
         rPx = x+vx*dt;
         rPy = y+vy*dt;
         rPz = z+vz*dt;
-        sz = surf_sag(s, rPx, rPy);
+        surf_sag_normal(s, rPx, rPy, sz, nx, ny, nz);
     }
-    return fabs(sz-rPz) <= kFastTol;
+    if (!(fabs(sz-rPz) <= kFastTol)) return false;
+    *memo = NormalMemo{dt, nx, ny, nz, true};
+    return true;
 }
 
-// Intersection with a fast start.  BT_START_CONIC seeds Newton with the base
-// conic's root; BT_START_MLP with `seed` (the caller's conic root plus the
-// network's correction, NaN when the conic has no root).  The reference path
-// decides every ray the fast start does not settle.
+// Intersection with a fast start: Newton from the base conic's root (the
+// ray origin where the surface has no conic base) plus `correction` (the
+// network's prediction for BT_START_MLP, 0 for BT_START_CONIC).  The
+// reference path decides every ray the fast start does not settle.
 BT_DEV bool fast_tti(const bt_surface* s, double x, double y, double z, double vx,
-                     double vy, double vz, double& dt, int niter, int start, double seed)
+                     double vy, double vz, double& dt, int niter, double correction,
+                     NormalMemo* memo)
 {
     if (newton_surface(s)) {
-        double d0 = seed;
-        bool ok = true;
-        if (start == BT_START_CONIC) ok = conic_start(s, x, y, z, vx, vy, vz, d0);
-        if (ok && d0 == d0 && newton_fast(s, x, y, z, vx, vy, vz, d0)) {
+        double d0;
+        const bool ok = conic_start(s, x, y, z, vx, vy, vz, d0);
+        d0 += correction;
+        if (ok && d0 == d0 && newton_fast(s, x, y, z, vx, vy, vz, d0, memo)) {
             dt = d0;
             return true;
         }
     }
     dt = 0.0;
-    return surf_tti(s, x, y, z, vx, vy, vz, dt, niter);
+    return surf_tti(s, x, y, z, vx, vy, vz, dt, niter, memo);
 }
 
 // ---------------------------------------------------------------------------
@@ -705,7 +792,7 @@ BT_DEV bool apply_interaction(int action, const double* __restrict__ dr,
                               const double* __restrict__ drot,
                               const bt_surface* surf, double n2,
                               Ray& r, int niter, int start = BT_START_REFERENCE,
-                              double seed = 0.0)
+                              double correction = 0.0)
 {
     if (r.fail) return false;
     double dx = r.x-dr[0];
@@ -723,7 +810,7 @@ BT_DEV bool apply_interaction(int action, const double* __restrict__ dr,
     memo.ok = false;
     const bool hit = (start == BT_START_REFERENCE)
                          ? surf_tti(surf, x, y, z, vx, vy, vz, dt, niter, &memo)
-                         : fast_tti(surf, x, y, z, vx, vy, vz, dt, niter, start, seed);
+                         : fast_tti(surf, x, y, z, vx, vy, vz, dt, niter, correction, &memo);
     // the normal at the intersection, reusing Newton's when it was evaluated
     // at this very point
     auto normal_at = [&](double px, double py, double& nx, double& ny, double& nz) {

@@ -65,49 +65,41 @@ __device__ __forceinline__ float mlp_predict_wave(const MlpPacked* __restrict__ 
     x[7] = (_Float16)0.0f;
     h8 zero;
     for (int k = 0; k < 8; k++) zero[k] = (_Float16)0.0f;
-    // ray column block cb (the rays of lane half cb), one at a time to keep
-    // few accumulators live
     float part[2] = {0.f, 0.f};
-    for (int cb = 0; cb < 2; cb++) {
-        // Reload the weights for each column block rather than holding all of
-        // them across both: an opaque (zero) byte offset keeps the compiler
-        // from hoisting the loads out of this loop.
-        int off;
-        asm volatile("s_mov_b32 %0, 0" : "=s"(off));
-        P = reinterpret_cast<const MlpPacked*>(reinterpret_cast<const char*>(P) + off);
-        const h8 b1 = (h == cb) ? x : zero;
-        // layer 1: H1^T[64 neurons][32 rays] as two 32x32 row blocks; relu(H1)
-        // in the accumulator layout is the layer-2 B operand
-        h8 hb[2][2];   // [rb1][t]
-        for (int rb = 0; rb < 2; rb++) {
-            const h8 a = *reinterpret_cast<const h8*>(P->w1 + (rb * 64 + lane) * 8);
+    const h8 b1c[2] = {(h == 0) ? x : zero, (h == 1) ? x : zero};
+    // layer 1 for both column blocks (the rays of lane half 0 and of lane
+    // half 1); relu(H1) in the accumulator layout is the layer-2 B operand.
+    // Each weight fragment is loaded once and used for both blocks.
+    h8 hb[2][2][2];   // [cb][rb1][t]
+    for (int rb = 0; rb < 2; rb++) {
+        const h8 a = *reinterpret_cast<const h8*>(P->w1 + (rb * 64 + lane) * 8);
+        for (int cb = 0; cb < 2; cb++) {
             f16v acc;
             for (int e = 0; e < 16; e++) acc[e] = 0.f;
-            acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b1, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b1c[cb], acc, 0, 0, 0);
             for (int t = 0; t < 2; t++)
                 for (int e = 0; e < 8; e++) {
                     const float v = acc[8 * t + e];
-                    hb[rb][t][e] = (_Float16)(v > 0.f ? v : 0.f);
+                    hb[cb][rb][t][e] = (_Float16)(v > 0.f ? v : 0.f);
                 }
         }
-        // layers 2 and 3, one output row block at a time
-        for (int rb2 = 0; rb2 < 2; rb2++) {
-            asm volatile("s_mov_b32 %0, 0" : "=s"(off));
-            P = reinterpret_cast<const MlpPacked*>(reinterpret_cast<const char*>(P) + off);
-            f16v acc;
-            const float* b2 = P->b2 + (rb2 * 64 + lane) * 16;
-            for (int e = 0; e < 16; e++) acc[e] = b2[e];
-            for (int s = 0; s < 4; s++) {
-                const h8 a = *reinterpret_cast<const h8*>(P->w2 + ((rb2 * 4 + s) * 64 + lane) * 8);
-                acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, hb[s >> 1][s & 1], acc, 0, 0, 0);
-            }
-            const float* w3 = P->w3 + (rb2 * 64 + lane) * 16;
-            float p = 0.f;
-            for (int e = 0; e < 16; e++) {
-                const float v = acc[e];
-                p = __builtin_fmaf(w3[e], v > 0.f ? v : 0.f, p);
-            }
-            part[cb] += p;
+    }
+    // layers 2 and 3, one output row block at a time
+    for (int rb2 = 0; rb2 < 2; rb2++) {
+        f16v acc0, acc1;
+        const float* b2 = P->b2 + (rb2 * 64 + lane) * 16;
+        for (int e = 0; e < 16; e++) { acc0[e] = b2[e]; acc1[e] = b2[e]; }
+        for (int s = 0; s < 4; s++) {
+            const h8 a = *reinterpret_cast<const h8*>(P->w2 + ((rb2 * 4 + s) * 64 + lane) * 8);
+            acc0 = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, hb[0][s >> 1][s & 1], acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, hb[1][s >> 1][s & 1], acc1, 0, 0, 0);
+        }
+        const float* w3 = P->w3 + (rb2 * 64 + lane) * 16;
+        for (int e = 0; e < 16; e++) {
+            const float w = w3[e];
+            const float v0 = acc0[e], v1 = acc1[e];
+            part[0] = __builtin_fmaf(w, v0 > 0.f ? v0 : 0.f, part[0]);
+            part[1] = __builtin_fmaf(w, v1 > 0.f ? v1 : 0.f, part[1]);
         }
     }
     // the two lane halves hold the two halves of each column's output rows

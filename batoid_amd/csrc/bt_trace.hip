@@ -172,10 +172,7 @@ __global__ __launch_bounds__(kBlock) void k_obscure(
     if (obsc_contains(obsc, obsc_len, rv.x[i], rv.y[i])) rv.vignetted[i] = 1;
 }
 
-// Every step of a plan applied while the ray stays in registers.  A ray's
-// wavelength never changes, so the index of each of the plan's media is
-// evaluated once per ray up front (at most kPlanMedia media; larger plans
-// evaluate the medium at each refraction).
+// Media indices evaluated once per ray (at most kPlanMedia per plan).
 constexpr int kPlanMedia = 4;
 
 __device__ __forceinline__ double pick_medium(const double (&nm)[kPlanMedia], int m)
@@ -189,12 +186,22 @@ __device__ __forceinline__ double pick_medium(const double (&nm)[kPlanMedia], in
     }
 }
 
-// The network's input for a ray at a BT_START_MLP step: its point on the base
-// conic and its direction in the step's local frame (the same arithmetic as
-// apply_interaction's frame change).  Returns the conic root, NaN if none.
-__device__ __forceinline__ double mlp_features(const bt_step* __restrict__ st,
-                                               const bt_surface* surf, const Ray& r,
-                                               float (&f)[kMlpIn])
+// Every step of a plan applied while the ray stays in registers.  A ray's
+// wavelength never changes, so the index of each of the plan's media is
+// evaluated once per ray up front (at most kPlanMedia media; larger plans
+// evaluate the medium at each refraction).
+//
+// Mode 0: reference starts only.  Mode 1: CONIC starts honoured (an MLP step
+// runs as CONIC).  Mode 2: MLP starts too; MLP inference is a wave-wide MFMA
+// product, so in this variant every lane of a wave stays in the step loop
+// (lanes past the end work on a copy of the last ray and store nothing).
+//
+// The network's input at an MLP step: the ray's point on the surface's base
+// conic and its direction, in the step's local frame (the frame change is
+// apply_interaction's).
+__device__ __forceinline__ void mlp_features(const bt_step* __restrict__ st,
+                                             const bt_surface* surf, const Ray& r,
+                                             float (&f)[kMlpIn])
 {
     const double* dr = st->dr;
     const double* m = st->drot;
@@ -206,35 +213,29 @@ __device__ __forceinline__ double mlp_features(const bt_step* __restrict__ st,
     double vy = r.vx*m[1] + r.vy*m[4] + r.vz*m[7];
     double vz = r.vx*m[2] + r.vy*m[5] + r.vz*m[8];
     double dtc = 0.0;
-    const bool ok = !r.fail && conic_start(surf, x, y, z, vx, vy, vz, dtc);
+    if (r.fail || !conic_start(surf, x, y, z, vx, vy, vz, dtc)) {
+        // keep this lane's share of the wave's evaluation finite
+        const MlpPacked* P = static_cast<const MlpPacked*>(st->mlp);
+        for (int k = 0; k < kMlpIn; k++) f[k] = P->norm[k];
+        return;
+    }
     f[0] = (float)(x + vx*dtc);
     f[1] = (float)(y + vy*dtc);
     f[2] = (float)(z + vz*dtc);
     f[3] = (float)vx;
     f[4] = (float)vy;
     f[5] = (float)vz;
-    if (!ok) {
-        // keep the (shared) network evaluation finite for this lane
-        const MlpPacked* P = static_cast<const MlpPacked*>(st->mlp);
-        for (int k = 0; k < kMlpIn; k++) f[k] = P->norm[k];
-    }
-    return ok ? dtc : __builtin_nan("");
 }
 
-// Every step of a plan applied while the ray stays in registers.  A ray's
-// wavelength never changes, so the index of each of the plan's media is
-// evaluated once per ray up front (at most kPlanMedia media; larger plans
-// evaluate the medium at each refraction).
-//
-// Fast: the plan has CONIC or MLP starts.  MLP inference is a wave-wide
-// MFMA product, so in this variant every lane of a wave stays in the step
-// loop (lanes past the end work on a copy of the last ray and store nothing).
-template <bool Fast>
-__global__ __launch_bounds__(kBlock) void k_plan(
+template <int Mode>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(Mode == 2 ? 3 : 1)))
+void k_plan(
     bt_plan plan, bt_rays in, bt_rays out, int niter, int need_w)
 {
+    constexpr bool Fast = Mode > 0;
+    constexpr bool Mlp = Mode > 1;
     const int64_t i0 = ray_index();
-    if (!Fast && i0 >= in.n) return;
+    if (!Mlp && i0 >= in.n) return;
     const bool live = i0 < in.n;
     const int64_t i = live ? i0 : in.n - 1;
     Ray r;
@@ -260,7 +261,7 @@ __global__ __launch_bounds__(kBlock) void k_plan(
     }
     const int nstep = plan.nstep;
     for (int k = 0; k < nstep; k++) {
-        if (!Fast && r.fail && r.vig) break;   // nothing below can change this ray
+        if (!Mlp && r.fail && r.vig) break;    // nothing below can change this ray
         const bt_step* __restrict__ st = plan.steps + k;
         const bt_surface* surf = plan.surfaces + st->surface;
         const int action = st->action;
@@ -269,21 +270,20 @@ __global__ __launch_bounds__(kBlock) void k_plan(
             const int mi = st->medium;
             n2 = cached ? pick_medium(nm, mi) : medium_n(plan.media + mi, r.w);
         }
-        double seed = 0.0;
+        double correction = 0.0;
         int start = Fast ? st->start : BT_START_REFERENCE;
-        if (start == BT_START_MLP && !st->mlp) start = BT_START_CONIC;
-        if (Fast && start == BT_START_MLP) {    // uniform: whole wave
+        if (start == BT_START_MLP && (!Mlp || !st->mlp)) start = BT_START_CONIC;
+        if (Mlp && start == BT_START_MLP) {     // uniform: whole wave
             float f[kMlpIn];
-            const double dtc = mlp_features(st, surf, r, f);
-            const float corr = mlp_predict_wave(static_cast<const MlpPacked*>(st->mlp), f);
-            seed = dtc + (double)corr;
+            mlp_features(st, surf, r, f);
+            correction = (double)mlp_predict_wave(static_cast<const MlpPacked*>(st->mlp), f);
         }
-        apply_interaction(action, st->dr, st->drot, surf, n2, r, niter, start, seed);
+        apply_interaction(action, st->dr, st->drot, surf, n2, r, niter, start, correction);
         const int olen = st->obsc_len;
         if (olen > 0 && !r.vig)
             r.vig = obsc_contains(plan.obscs + st->obsc, olen, r.x, r.y);
     }
-    if (Fast && !live) return;
+    if (Mlp && !live) return;
     out.x[i] = r.x; out.y[i] = r.y; out.z[i] = r.z;
     out.vx[i] = r.vx; out.vy[i] = r.vy; out.vz[i] = r.vz;
     out.t[i] = r.t;
@@ -501,11 +501,14 @@ int bt_trace_plan(bt_plan plan, bt_rays in, bt_rays out, int niter, void* stream
     if (niter < 0) return fail(BT_E_SIZE, "bt_trace_plan: negative niter");
     if (in.n == 0) return 0;
     const int need_w = (plan.flags & BT_PLAN_NO_REFRACT) ? 0 : 1;
-    if (plan.flags & BT_PLAN_FAST_START) {
-        hipLaunchKernelGGL(k_plan<true>, dim3(grid_for(in.n)), dim3(kBlock), 0,
+    if (plan.flags & BT_PLAN_MLP_START) {
+        hipLaunchKernelGGL(k_plan<2>, dim3(grid_for(in.n)), dim3(kBlock), 0,
+                           as_stream(stream), plan, in, out, niter, need_w);
+    } else if (plan.flags & BT_PLAN_FAST_START) {
+        hipLaunchKernelGGL(k_plan<1>, dim3(grid_for(in.n)), dim3(kBlock), 0,
                            as_stream(stream), plan, in, out, niter, need_w);
     } else {
-        hipLaunchKernelGGL(k_plan<false>, dim3(grid_for(in.n)), dim3(kBlock), 0,
+        hipLaunchKernelGGL(k_plan<0>, dim3(grid_for(in.n)), dim3(kBlock), 0,
                            as_stream(stream), plan, in, out, niter, need_w);
     }
     return check_launch("bt_trace_plan");

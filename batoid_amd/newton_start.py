@@ -3,21 +3,25 @@
 No counterpart in the reference.  The reference intersects a Newton surface
 (Asphere, Bicubic, Zernike/PolynomialSurface, Sum) by starting from an
 analytic root -- the Asphere's conic (asphere.cpp:75-85) or a Sum's first
-child (sum.cpp:47-56) -- and iterating Newton a fixed number of times
-(surface.cpp:22-49).  Here a small network
+child (sum.cpp:47-56), the ray origin for a polynomial or Bicubic surface
+(surface.cpp:22-49) -- and iterating Newton a fixed number of times.  Here a
+small network per Newton surface
 
     6 -> 64 -> 64 -> 1, ReLU
 
 predicts the residual ``dt* - dt_conic`` between the exact intersection time
-and the base conic's root, from the ray's point on that conic and its
-direction (the surface's local frame).  Training data comes from the tracer
-(``bt_newton_samples``), training runs on the GPU in FP32 with the library's
-own forward/backward and Adam kernels, and inference inside the fused trace
-is an fp16 MFMA product per wave.
+and the surface's base root (the conic's root; 0, i.e. the ray origin, for
+surfaces without a conic base) from the ray's point at that root and its
+direction, in the surface's local frame.  Inside the fused trace each
+network is an fp16 MFMA product evaluated by whole waves at its step.
+Training data come from the tracer (``bt_newton_samples``); training runs on
+the GPU in FP32 with the library's own forward/backward and Adam kernels.
 
 The start changes only where Newton begins: every ray is still converged on
 the full surface (to a sag gap of 1e-14 m) and any ray that does not
 converge from the learned start is re-intersected on the reference path.
+Networks are specific to the surface and the ray bundles they were trained
+on.
 
 Usage::
 

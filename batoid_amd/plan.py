@@ -68,7 +68,7 @@ class PlanSpec:
         self.interfaces = []
         surf_index, med_index, ob_index = {}, {}, {}
         refracts = False
-        fast = False
+        fast = mlp = False
         self.networks = []      # packed network buffers the steps point to
         cs = coordSys_in
         for iface, action, m2 in steps:
@@ -110,6 +110,7 @@ class PlanSpec:
                 self.networks.append(packed)
                 st.mlp = packed.data_ptr()
             fast = fast or mode != _lib.START_REFERENCE
+            mlp = mlp or mode == _lib.START_MLP
             for i, v in enumerate(ct.dr):
                 st.dr[i] = float(v)
             for i, v in enumerate(np.asarray(ct.drot).ravel()):
@@ -119,8 +120,9 @@ class PlanSpec:
             cs = iface.coordSys
         self.coordSys_in = coordSys_in
         self.coordSys_out = cs
-        self.flags = (0 if refracts else _lib.PLAN_NO_REFRACT) | (
-            _lib.PLAN_FAST_START if fast else 0)
+        self.flags = ((0 if refracts else _lib.PLAN_NO_REFRACT)
+                      | (_lib.PLAN_FAST_START if fast else 0)
+                      | (_lib.PLAN_MLP_START if mlp else 0))
 
 
 class DevicePlan:

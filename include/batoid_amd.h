@@ -163,7 +163,8 @@ enum bt_action { BT_ACT_INTERSECT = 0, BT_ACT_REFLECT = 1, BT_ACT_REFRACT = 2 };
  * A ray that does not converge from a fast start is re-intersected on the
  * REFERENCE path, so failure semantics stay the reference's.  Starts other
  * than REFERENCE are honoured only by bt_trace_plan with BT_PLAN_FAST_START
- * set; an MLP step with a null network runs as CONIC. */
+ * set; MLP starts only with BT_PLAN_MLP_START also set, and an MLP step with
+ * a null network runs as CONIC. */
 enum bt_start { BT_START_REFERENCE = 0, BT_START_CONIC = 1, BT_START_MLP = 2 };
 
 typedef struct bt_step {
@@ -181,6 +182,8 @@ typedef struct bt_step {
 /* bt_plan.flags */
 #define BT_PLAN_NO_REFRACT 1   /* no step refracts: wavelength is not read */
 #define BT_PLAN_FAST_START 2   /* some step uses a CONIC or MLP start      */
+#define BT_PLAN_MLP_START  4   /* some step uses an MLP start; without this
+                                  flag MLP steps run as CONIC             */
 
 typedef struct bt_plan {
     int32_t nstep;
@@ -287,19 +290,22 @@ int bt_sum_amplitude(bt_rays rays, double x, double y, double z, double t,
                      void* stream);
 
 /* ---- learned Newton start ------------------------------------------------
- * No counterpart in the reference: the network that seeds BT_START_MLP
+ * No counterpart in the reference: the networks that seed BT_START_MLP
  * steps, 6 -> 64 -> 64 -> 1 with ReLU, trained in FP32 on intersections
- * the tracer produces.  Parameters are one FP32 vector of
- * bt_mlp_param_count() floats: W1[64][6], b1[64], W2[64][64], b2[64],
- * w3[64], b3.  `in_norm` is in_mean[6], in_inv_std[6] (14 floats with
- * out_mean, out_std for bt_mlp_pack).  Features are [n][6] FP32: the ray's
- * point on the surface's base conic and its direction, local frame; the
- * target is (dt_exact - dt_conic - out_mean) / out_std.                     */
+ * the tracer produces.  A network reads, in its step's local frame, the
+ * ray's point on the surface's base conic (the ray origin where the surface
+ * has no conic base) and its direction, and predicts (dt* - dt_conic): the
+ * exact intersection time minus the base root.  Parameters are one FP32
+ * vector of bt_mlp_param_count() floats: W1[64][6], b1[64], W2[64][64],
+ * b2[64], w3[64], b3.  `in_norm` is in_mean[6], in_inv_std[6] (14 floats
+ * with out_mean, out_std for bt_mlp_pack).  Features are [n][6] FP32;
+ * targets are (dt* - dt_conic - out_mean) / out_std.                        */
 int bt_mlp_param_count(void);
 int64_t bt_mlp_packed_bytes(void);
 int64_t bt_mlp_grad_workspace_bytes(int64_t batch);
 /* Mean squared error over `batch` samples (rows index[b] of features /
- * targets, or rows 0..batch-1 when index is NULL) and its gradient. */
+ * targets, or rows 0..batch-1 when index is NULL; every index must be a
+ * valid row) and its gradient. */
 int bt_mlp_loss_grad(const float* params, const float* features, const float* targets,
                      const int32_t* index, int64_t batch, const float* in_norm,
                      float* grad, float* loss, void* workspace, int64_t workspace_bytes,
@@ -315,9 +321,10 @@ int bt_mlp_pack(const float* params, const float* norm, void* packed, void* stre
 /* Packed (MFMA) inference, output de-normalised (metres). */
 int bt_mlp_forward_packed(const void* packed, const float* features, int64_t n,
                           float* out, void* stream);
-/* Training samples from the tracer: for each ray, the local frame of the
- * step, the base-conic start, the reference intersection (niter Newton
- * iterations) and status bits (1: conic start ok, 2: intersection ok). */
+/* Training samples from the tracer: for rays about to meet `surface` (frame
+ * change dr, drot), the network features, the base-conic root, the
+ * reference intersection (niter Newton iterations) and status bits (1:
+ * conic root found, 2: intersection converged). */
 int bt_newton_samples(const bt_surface* surface, const double* dr, const double* drot,
                       bt_rays rays, int niter, float* features, double* dt_conic,
                       double* dt_exact, uint8_t* status, void* stream);
