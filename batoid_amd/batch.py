@@ -124,6 +124,13 @@ class PSFBatch:
         self.graph = None
         self._configure(chunk, nwg)
 
+    def set_start(self, start):
+        """Change the trace's Newton start (see CompoundOptic.trace); a
+        captured graph is dropped."""
+        self.start = start
+        self.plan = self.optic._plan(self.rays_in.coordSys, False, start)
+        self.graph = None
+
     # ------------------------------------------------------------------
     def _configure(self, chunk, nwg):
         lib = _lib.load()

@@ -45,7 +45,7 @@ typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 
 constexpr double kTwoPi = 2 * 3.14159265358979323846;
 constexpr int kTile = 128;          // workgroup tile (complex) in i and j
-constexpr int kRayStride = 8;       // floats per prepared ray
+constexpr int kRayStride = 16;      // floats per prepared ray
 constexpr int kRayGroup = 8;        // rays per bf16 MFMA step; splits align to it
 constexpr int kScanThreads = 1024;
 
@@ -108,8 +108,11 @@ __global__ __launch_bounds__(kScanThreads) void k_psf_scan(const int* __restrict
     if (threadIdx.x == kScanThreads - 1) offsets[nb] = s[kScanThreads - 1];
 }
 
-// Per ray: phase at p0 (revolutions) and per-pixel phase steps
-// [alpha_hi, alpha_lo, beta_hi, beta_lo, phi0_rev, flux, 0, 0].
+// Per ray: phase at p0 (revolutions), per-pixel phase steps and the
+// rotations between the row blocks of a wave's tile (16, 32, 48 rows apart,
+// used by the split-BF16 path to derive blocks 1-3 from block 0):
+// [alpha_hi, alpha_lo, beta_hi, beta_lo, phi0_rev, flux,
+//  cos, sin of 2 pi frac(16 k alpha) for k = 1, 2, 3, 0, 0, 0, 0].
 __device__ __forceinline__ void prep_ray(const bt_rays& rv, int64_t i, double p0x, double p0y,
                                          double tref, double a1x, double a1y, double a2x,
                                          double a2y, int keep_a, int keep_b,
@@ -140,8 +143,16 @@ __device__ __forceinline__ void prep_ray(const bt_rays& rv, int64_t i, double p0
     o[3] = (float)(be - (double)bh);
     o[4] = (float)rev;
     o[5] = (float)rv.flux[i];
-    o[6] = 0.f;
-    o[7] = 0.f;
+    for (int k = 1; k <= 3; k++) {
+        double d = 16.0 * k * al;
+        d -= floor(d);
+        o[4 + 2*k] = (float)cos(kTwoPi * d);
+        o[5 + 2*k] = (float)sin(kTwoPi * d);
+    }
+    o[12] = 0.f;
+    o[13] = 0.f;
+    o[14] = 0.f;
+    o[15] = 0.f;
 }
 
 // The parameters of every valid ray, compacted in ray order.
@@ -251,9 +262,16 @@ __device__ __forceinline__ void psf_tile(const float* __restrict__ params, int r
                 const float* p = params + (int64_t)(live ? rr : r0) * kRayStride;
                 const float ah = p[0], al = p[1], bh = p[2], bl = p[3], ph0 = p[4];
                 const float f = live ? p[5] : 0.f;
+                // row block 0 evaluated, blocks 1-3 rotated from it
+                const float th = step_phase(ci[0], ah, al, ph0);
+                const float fs0 = f * sin_rev(th), fc0 = f * cos_rev(th);
                 for (int rb = 0; rb < 4; rb++) {
-                    const float th = step_phase(ci[rb], ah, al, ph0);
-                    const float s = f * sin_rev(th), c = f * cos_rev(th);
+                    float s = fs0, c = fc0;
+                    if (rb > 0) {
+                        const float cr = p[4 + 2*rb], sr = p[5 + 2*rb];
+                        s = __builtin_fmaf(fs0, cr, fc0 * sr);
+                        c = __builtin_fmaf(fc0, cr, -(fs0 * sr));
+                    }
                     const float v0 = prow ? s : c;        // k part 0
                     const float v1 = prow ? c : -s;       // k part 1
                     const __bf16 h0 = (__bf16)v0, h1 = (__bf16)v1;

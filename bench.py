@@ -299,6 +299,23 @@ def bench_batch(reps=5):
            "rel_l2_vs_fp64_direct": errs}
     log(f"batch: {graph_ms:.2f} ms per batch (graph), {eager_ms:.2f} ms eager, trace "
         f"{trace_ms:.2f} ms, PSF {res['psf_tflops']:.1f} TFLOP/s, rel L2 {max(errs):.1e}")
+    # the same batch with the conic Newton start (positions within 1e-12 m of
+    # the reference start, flags identical)
+    ref_x = b.rays.torch("x").clone()
+    ref_fail = b.rays.torch("failed").clone()
+    b.set_start("conic")
+    b.capture()
+    conic_ms = b.time(reps)
+    torch.cuda.synchronize()
+    ok = ref_fail == 0
+    res["conic_start"] = {
+        "ms_per_batch_graph": conic_ms,
+        "rays_per_s": b.n / (conic_ms * 1e-3),
+        "ray_surfaces_per_s": b.n * nsteps / (conic_ms * 1e-3),
+        "max_abs_dx_m": float((b.rays.torch("x") - ref_x)[ok].abs().max()),
+        "flag_mismatches": int((b.rays.torch("failed") != ref_fail).sum()),
+    }
+    log(f"batch, conic start: {conic_ms:.2f} ms per batch (graph)")
     return res
 
 
