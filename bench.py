@@ -34,10 +34,10 @@ BYTES_PER_RAY = 8 * 8 + 2 + 7 * 8 + 2
 # rocprofv3 FETCH_SIZE and WRITE_SIZE (each scaled by its reading for a
 # 1 GiB device copy: x2.000 and x1.000), from
 # profiles/trace_plan_tma_1e7_counters.md.
-MEASURED_TRAFFIC_PER_RAY = 124.04
+MEASURED_TRAFFIC_PER_RAY = 124.06
 # FP64 flops per ray of the same kernel on this workload: 64 lanes x
 # (ADD + MUL + 2 FMA) FP64 wave instructions (SQ_INSTS_VALU_*_F64), same file.
-FP64_FLOP_PER_RAY = 8271.0
+FP64_FLOP_PER_RAY = 7024.0
 
 
 def log(*a):
