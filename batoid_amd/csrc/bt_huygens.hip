@@ -45,7 +45,8 @@ typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 
 constexpr double kTwoPi = 2 * 3.14159265358979323846;
 constexpr int kTile = 128;          // workgroup tile (complex) in i and j
-constexpr int kRayStride = 16;      // floats per prepared ray
+constexpr int kRayStride = 8;       // floats per prepared ray
+constexpr int kRotStride = 8;       // floats per ray of row-block rotations
 constexpr int kRayGroup = 8;        // rays per bf16 MFMA step; splits align to it
 constexpr int kScanThreads = 1024;
 
@@ -108,15 +109,15 @@ __global__ __launch_bounds__(kScanThreads) void k_psf_scan(const int* __restrict
     if (threadIdx.x == kScanThreads - 1) offsets[nb] = s[kScanThreads - 1];
 }
 
-// Per ray: phase at p0 (revolutions), per-pixel phase steps and the
-// rotations between the row blocks of a wave's tile (16, 32, 48 rows apart,
-// used by the split-BF16 path to derive blocks 1-3 from block 0):
-// [alpha_hi, alpha_lo, beta_hi, beta_lo, phi0_rev, flux,
-//  cos, sin of 2 pi frac(16 k alpha) for k = 1, 2, 3, 0, 0, 0, 0].
+// Per ray: phase at p0 (revolutions) and per-pixel phase steps
+//   o   = [alpha_hi, alpha_lo, beta_hi, beta_lo, phi0_rev, flux, 0, 0]
+// and the rotations between the row blocks of a wave's tile (16, 32, 48
+// rows apart), with which the split-BF16 path derives blocks 1-3 from 0
+//   rot = [cos, sin of 2 pi frac(16 k alpha) for k = 1, 2, 3, 0, 0].
 __device__ __forceinline__ void prep_ray(const bt_rays& rv, int64_t i, double p0x, double p0y,
                                          double tref, double a1x, double a1y, double a2x,
                                          double a2y, int keep_a, int keep_b,
-                                         float* __restrict__ o)
+                                         float* __restrict__ o, float* __restrict__ rot)
 {
     const double x = rv.x[i], y = rv.y[i], z = rv.z[i];
     const double vx = rv.vx[i], vy = rv.vy[i], vz = rv.vz[i];
@@ -143,22 +144,23 @@ __device__ __forceinline__ void prep_ray(const bt_rays& rv, int64_t i, double p0
     o[3] = (float)(be - (double)bh);
     o[4] = (float)rev;
     o[5] = (float)rv.flux[i];
+    o[6] = 0.f;
+    o[7] = 0.f;
     for (int k = 1; k <= 3; k++) {
         double d = 16.0 * k * al;
         d -= floor(d);
-        o[4 + 2*k] = (float)cos(kTwoPi * d);
-        o[5 + 2*k] = (float)sin(kTwoPi * d);
+        rot[2*k - 2] = (float)cos(kTwoPi * d);
+        rot[2*k - 1] = (float)sin(kTwoPi * d);
     }
-    o[12] = 0.f;
-    o[13] = 0.f;
-    o[14] = 0.f;
-    o[15] = 0.f;
+    rot[6] = 0.f;
+    rot[7] = 0.f;
 }
 
 // The parameters of every valid ray, compacted in ray order.
 __global__ __launch_bounds__(kBlock) void k_psf_prep(bt_rays rv, PsfGeom g, int ignore_vig,
                                                       const int* __restrict__ offsets,
-                                                      float* __restrict__ params)
+                                                      float* __restrict__ params,
+                                                      float* __restrict__ rot)
 {
     __shared__ int s[kBlock / 64];
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -172,7 +174,7 @@ __global__ __launch_bounds__(kBlock) void k_psf_prep(bt_rays rv, PsfGeom g, int 
     for (int k = 0; k < w; k++) base += s[k];
     const int rank = base + __popcll(m & ((1ull << lane) - 1ull));
     prep_ray(rv, i, g.p0x, g.p0y, g.t, g.a1x, g.a1y, g.a2x, g.a2y, g.keep_a, g.keep_b,
-             params + (int64_t)rank * kRayStride);
+             params + (int64_t)rank * kRayStride, rot + (int64_t)rank * kRotStride);
 }
 
 // sin(2 pi theta) with theta in revolutions (v_sin_f32).
@@ -189,7 +191,8 @@ __device__ __forceinline__ float step_phase(float c, float hi, float lo, float a
 // the prepared parameters, written as FP32 (re, im) into out[i][j][2] (the
 // padded nxp x nyp image).  Every lane of the workgroup must call it.
 template <int kPrec>
-__device__ __forceinline__ void psf_tile(const float* __restrict__ params, int r0, int r1,
+__device__ __forceinline__ void psf_tile(const float* __restrict__ params,
+                                         const float* __restrict__ rot, int r0, int r1,
                                          int tile_i0, int tile_j0, int nx, int ny, int nyp,
                                          float* __restrict__ out)
 {
@@ -268,7 +271,8 @@ __device__ __forceinline__ void psf_tile(const float* __restrict__ params, int r
                 for (int rb = 0; rb < 4; rb++) {
                     float s = fs0, c = fc0;
                     if (rb > 0) {
-                        const float cr = p[4 + 2*rb], sr = p[5 + 2*rb];
+                        const float* pr = rot + (int64_t)(live ? rr : r0) * kRotStride;
+                        const float cr = pr[2*rb - 2], sr = pr[2*rb - 1];
                         s = __builtin_fmaf(fs0, cr, fc0 * sr);
                         c = __builtin_fmaf(fc0, cr, -(fs0 * sr));
                     }
@@ -316,9 +320,9 @@ __device__ __forceinline__ void psf_tile(const float* __restrict__ params, int r
 
 template <int kPrec>
 __global__ __launch_bounds__(256, 2) void k_psf_gemm(
-    const float* __restrict__ params, const int* __restrict__ total_ptr,
-    int nx, int ny, int tiles_i, int ksplit, int nxp, int nyp,
-    float* __restrict__ partial)
+    const float* __restrict__ params, const float* __restrict__ rot,
+    const int* __restrict__ total_ptr, int nx, int ny, int tiles_i, int ksplit, int nxp,
+    int nyp, float* __restrict__ partial)
 {
     const int tile = blockIdx.x;
     const int ks = blockIdx.y;
@@ -327,7 +331,7 @@ __global__ __launch_bounds__(256, 2) void k_psf_gemm(
     per = ((per + kRayGroup - 1) / kRayGroup) * kRayGroup;
     const int r0 = ks * per;
     const int r1 = min(total, r0 + per);
-    psf_tile<kPrec>(params, r0, r1, (tile % tiles_i) * kTile, (tile / tiles_i) * kTile, nx, ny,
+    psf_tile<kPrec>(params, rot, r0, r1, (tile % tiles_i) * kTile, (tile / tiles_i) * kTile, nx, ny,
                     nyp, partial + (int64_t)ks * nxp * nyp * 2);
 }
 
@@ -441,7 +445,7 @@ __device__ __forceinline__ int find_segment(const bt_psf_segment* __restrict__ s
 __global__ __launch_bounds__(kBlock) void k_psf_prep_batch(
     bt_rays rv, const bt_psf_segment* __restrict__ segs, int nseg,
     const SegGeom* __restrict__ geom, int keep_a, int keep_b, int ignore_vig,
-    const int* __restrict__ offsets, float* __restrict__ params)
+    const int* __restrict__ offsets, float* __restrict__ params, float* __restrict__ rot)
 {
     __shared__ int s[kBlock / 64];
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -455,14 +459,17 @@ __global__ __launch_bounds__(kBlock) void k_psf_prep_batch(
     for (int k = 0; k < w; k++) base += s[k];
     const int rank = base + __popcll(m & ((1ull << lane) - 1ull));
     float* o = params + (int64_t)rank * kRayStride;
+    float* orot = rot + (int64_t)rank * kRotStride;
     const int sg = find_segment(segs, nseg, i);
     if (sg < 0) {                       // outside every segment: never summed
         for (int k = 0; k < kRayStride; k++) o[k] = 0.f;
+        for (int k = 0; k < kRotStride; k++) orot[k] = 0.f;
         return;
     }
     const SegGeom g = geom[sg];
     const bt_psf_segment& d = segs[sg];
-    prep_ray(rv, i, g.p0x, g.p0y, g.t, d.a1[0], d.a1[1], d.a2[0], d.a2[1], keep_a, keep_b, o);
+    prep_ray(rv, i, g.p0x, g.p0y, g.t, d.a1[0], d.a1[1], d.a2[0], d.a2[1], keep_a, keep_b, o,
+             orot);
 }
 
 // Number of valid rays before ray index b (b <= n), from the block offsets.
@@ -550,7 +557,8 @@ __global__ __launch_bounds__(kScanThreads) void k_psf_items(SegRange* __restrict
 // fixed assignment, so the result does not depend on scheduling.
 template <int kPrec>
 __global__ __launch_bounds__(256, 2) void k_psf_gemm_batch(
-    const float* __restrict__ params, const WorkItem* __restrict__ items,
+    const float* __restrict__ params, const float* __restrict__ rot,
+    const WorkItem* __restrict__ items,
     const int* __restrict__ nitems_ptr, int tiles_i, int tiles, int nx, int ny, int nxp,
     int nyp, float* __restrict__ partial)
 {
@@ -559,7 +567,7 @@ __global__ __launch_bounds__(256, 2) void k_psf_gemm_batch(
         const int item = (int)(wk / tiles);
         const int tile = (int)(wk % tiles);
         const WorkItem it = items[item];
-        psf_tile<kPrec>(params, it.r0, it.r1, (tile % tiles_i) * kTile, (tile / tiles_i) * kTile,
+        psf_tile<kPrec>(params, rot, it.r0, it.r1, (tile % tiles_i) * kTile, (tile / tiles_i) * kTile,
                         nx, ny, nyp, partial + (int64_t)item * nxp * nyp * 2);
     }
 }
@@ -616,7 +624,8 @@ GemmLayout gemm_layout(int64_t n, int nx, int ny)
     L.ksplit = ks;
     auto align = [](int64_t v) { return (v + 255) & ~(int64_t)255; };
     int64_t off = 0;
-    L.off_params = off;  off = align(off + (n + kRayGroup) * kRayStride * (int64_t)sizeof(float));
+    L.off_params = off;
+    off = align(off + (n + kRayGroup) * (kRayStride + kRotStride) * (int64_t)sizeof(float));
     L.off_counts = off;  off = align(off + L.nblocks * (int64_t)sizeof(int));
     L.off_offsets = off; off = align(off + (L.nblocks + 1) * (int64_t)sizeof(int));
     L.off_partial = off; off = align(off + (int64_t)ks * L.nxp * L.nyp * 2 * (int64_t)sizeof(float));
@@ -653,6 +662,7 @@ extern "C" int bt_huygens_gemm(bt_rays rays, const double* p0, const double* a1,
     if (workspace_bytes < L.bytes) return fail(BT_E_SIZE, "bt_huygens_gemm: workspace too small");
     char* ws = static_cast<char*>(workspace);
     float* params = reinterpret_cast<float*>(ws + L.off_params);
+    float* rot = params + (rays.n + kRayGroup) * kRayStride;
     int* counts = reinterpret_cast<int*>(ws + L.off_counts);
     int* offsets = reinterpret_cast<int*>(ws + L.off_offsets);
     float* partial = reinterpret_cast<float*>(ws + L.off_partial);
@@ -682,17 +692,17 @@ extern "C" int bt_huygens_gemm(bt_rays rays, const double* p0, const double* a1,
     if (rc) return rc;
     if (rays.n > 0) {
         hipLaunchKernelGGL(k_psf_prep, dim3(nb), dim3(kBlock), 0, s, rays, g, ignore_vignetted,
-                           offsets, params);
+                           offsets, params, rot);
         rc = check_launch("bt_huygens_gemm(prep)");
         if (rc) return rc;
     }
     const int* total = offsets + (rays.n > 0 ? nb : 0);
     dim3 grid(L.tiles_i * L.tiles_j, L.ksplit);
     if (precision == BT_PSF_FP32)
-        hipLaunchKernelGGL(k_psf_gemm<BT_PSF_FP32>, grid, dim3(256), 0, s, params, total, nx, ny,
+        hipLaunchKernelGGL(k_psf_gemm<BT_PSF_FP32>, grid, dim3(256), 0, s, params, rot, total, nx, ny,
                            L.tiles_i, L.ksplit, L.nxp, L.nyp, partial);
     else
-        hipLaunchKernelGGL(k_psf_gemm<BT_PSF_BF16X3>, grid, dim3(256), 0, s, params, total, nx, ny,
+        hipLaunchKernelGGL(k_psf_gemm<BT_PSF_BF16X3>, grid, dim3(256), 0, s, params, rot, total, nx, ny,
                            L.tiles_i, L.ksplit, L.nxp, L.nyp, partial);
     rc = check_launch("bt_huygens_gemm(mfma)");
     if (rc) return rc;
@@ -728,7 +738,8 @@ BatchLayout batch_layout(int64_t n, int nseg, int nx, int ny, int chunk)
     L.off_ranges = off;  off = align(off + nseg * (int64_t)sizeof(SegRange));
     L.off_items = off;   off = align(off + L.cap * (int64_t)sizeof(WorkItem));
     L.off_nitems = off;  off = align(off + (int64_t)sizeof(int));
-    L.off_params = off;  off = align(off + (n + kRayGroup) * kRayStride * (int64_t)sizeof(float));
+    L.off_params = off;
+    off = align(off + (n + kRayGroup) * (kRayStride + kRotStride) * (int64_t)sizeof(float));
     L.off_partial = off; off = align(off + L.cap * L.nxp * L.nyp * 2 * (int64_t)sizeof(float));
     L.bytes = off;
     return L;
@@ -774,6 +785,7 @@ extern "C" int bt_huygens_batch(bt_rays rays, const bt_psf_segment* segments, in
     WorkItem* items = reinterpret_cast<WorkItem*>(ws + L.off_items);
     int* nitems = reinterpret_cast<int*>(ws + L.off_nitems);
     float* params = reinterpret_cast<float*>(ws + L.off_params);
+    float* rot = params + (rays.n + kRayGroup) * kRayStride;
     float* partial = reinterpret_cast<float*>(ws + L.off_partial);
     hipStream_t s = as_stream(stream);
 
@@ -798,7 +810,7 @@ extern "C" int bt_huygens_batch(bt_rays rays, const bt_psf_segment* segments, in
     if (rc) return rc;
     if (rays.n > 0) {
         hipLaunchKernelGGL(k_psf_prep_batch, dim3(nb), dim3(kBlock), 0, s, rays, segments, nseg,
-                           geom, keep_a, keep_b, ignore_vignetted, offsets, params);
+                           geom, keep_a, keep_b, ignore_vignetted, offsets, params, rot);
         rc = check_launch("bt_huygens_batch(prep)");
         if (rc) return rc;
     }
@@ -814,10 +826,10 @@ extern "C" int bt_huygens_batch(bt_rays rays, const bt_psf_segment* segments, in
     const int tiles = L.tiles_i * L.tiles_j;
     if (precision == BT_PSF_FP32)
         hipLaunchKernelGGL(k_psf_gemm_batch<BT_PSF_FP32>, dim3(grid), dim3(256), 0, s, params,
-                           items, nitems, L.tiles_i, tiles, nx, ny, L.nxp, L.nyp, partial);
+                           rot, items, nitems, L.tiles_i, tiles, nx, ny, L.nxp, L.nyp, partial);
     else
         hipLaunchKernelGGL(k_psf_gemm_batch<BT_PSF_BF16X3>, dim3(grid), dim3(256), 0, s, params,
-                           items, nitems, L.tiles_i, tiles, nx, ny, L.nxp, L.nyp, partial);
+                           rot, items, nitems, L.tiles_i, tiles, nx, ny, L.nxp, L.nyp, partial);
     rc = check_launch("bt_huygens_batch(mfma)");
     if (rc) return rc;
     const int64_t red_blocks = (((int64_t)nx * ny + kRedPix - 1) / kRedPix) * nseg;
