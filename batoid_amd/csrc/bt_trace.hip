@@ -227,8 +227,14 @@ __device__ __forceinline__ void mlp_features(const bt_step* __restrict__ st,
     f[5] = (float)vz;
 }
 
+// Waves per SIMD: mode 0 is held to four (some registers spill to scratch;
+// the extra wave hides more FP64 latency: 8 % faster on the TMA, 9-12 % on
+// the Newton-heavy optics), mode 2 to three (left alone the compiler would
+// settle for two); mode 1 keeps the compiler's three, where four measured
+// slower on the TMA.
 template <int Mode>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(Mode == 2 ? 3 : 1)))
+__global__ __launch_bounds__(kBlock)
+__attribute__((amdgpu_waves_per_eu(Mode == 0 ? 4 : (Mode == 2 ? 3 : 1))))
 void k_plan(
     bt_plan plan, bt_rays in, bt_rays out, int niter, int need_w)
 {
