@@ -89,6 +89,8 @@ START_REFERENCE, START_CONIC, START_MLP = range(3)
 PLAN_NO_REFRACT = 1
 PLAN_FAST_START = 2
 PLAN_MLP_START = 4
+PLAN_ANALYTIC = 8
+ANALYTIC_KINDS = (PLANE, SPHERE, PARABOLOID, QUADRIC, ASPHERE, TILTED)
 PSF_FP32, PSF_BF16X3 = 0, 1
 OBSC_MAX_STACK = 32
 MAX_SUM_CHILDREN = 8

@@ -215,8 +215,19 @@ BT_DEV double asphere_dzdr(const bt_surface* s, double r)
 }
 
 // ---------------------------------------------------------------------------
+// Surface kinds a kernel is compiled for.  kKindsAnalytic covers Plane,
+// Sphere, Paraboloid, Quadric, Asphere and Tilted surfaces without Sums: the
+// Bicubic, polynomial and Sum code paths are left out, which lets a plan
+// kernel for such optics keep far fewer registers.  Under it, any other kind
+// evaluates to NaN (and so fails).
+// ---------------------------------------------------------------------------
+constexpr int kKindsAll = 0;
+constexpr int kKindsAnalytic = 1;
+
+// ---------------------------------------------------------------------------
 // Leaf surfaces: sag and normal
 // ---------------------------------------------------------------------------
+template <int K = kKindsAll>
 BT_DEV double leaf_sag(const bt_surface* s, double x, double y)
 {
     switch (s->kind) {
@@ -249,13 +260,17 @@ BT_DEV double leaf_sag(const bt_surface* s, double x, double y)
     case BT_TILTED:                                  // tilted.cpp:16-18
         return x*s->p[0] + y*s->p[1];
     case BT_BICUBIC:                                 // bicubic.cpp:19-21
-        return table_eval(s, x, y);
+        if constexpr (K == kKindsAll) return table_eval(s, x, y);
+        break;
     case BT_POLYNOMIAL:                              // polynomialSurface.cpp:35-37
-        return horner2d(x-s->p[0], y-s->p[1], s->a0, s->n1, s->n2);
+        if constexpr (K == kKindsAll)
+            return horner2d(x-s->p[0], y-s->p[1], s->a0, s->n1, s->n2);
+        break;
     }
     return __builtin_nan("");
 }
 
+template <int K = kKindsAll>
 BT_DEV void leaf_normal(const bt_surface* s, double x, double y,
                         double& nx, double& ny, double& nz)
 {
@@ -309,50 +324,56 @@ BT_DEV void leaf_normal(const bt_surface* s, double x, double y,
         nz = sqrt(1 - tx*tx - ty*ty);
         return;
     }
-    case BT_BICUBIC: {                               // bicubic.cpp:23-40
-        double dxdz, dydz;
-        table_grad(s, x, y, dxdz, dydz);
-        if (__builtin_isnan(dxdz)) {
-            nx = __builtin_nan(""); ny = nx; nz = nx;
+    case BT_BICUBIC:                                 // bicubic.cpp:23-40
+        if constexpr (K == kKindsAll) {
+            double dxdz, dydz;
+            table_grad(s, x, y, dxdz, dydz);
+            if (__builtin_isnan(dxdz)) {
+                nx = __builtin_nan(""); ny = nx; nz = nx;
+                return;
+            }
+            nz = 1/sqrt(1 + dxdz*dxdz + dydz*dydz);
+            nx = -dxdz*nz;
+            ny = -dydz*nz;
             return;
         }
-        nz = 1/sqrt(1 + dxdz*dxdz + dydz*dydz);
-        nx = -dxdz*nz;
-        ny = -dydz*nz;
-        return;
-    }
-    case BT_POLYNOMIAL: {                            // polynomialSurface.cpp:39-48
-        double gx = -horner2d(x-s->p[0], y-s->p[1], s->a1, s->n1-1, s->n2-1);
-        double gy = -horner2d(x-s->p[0], y-s->p[1], s->a2, s->n1-1, s->n2-1);
-        nz = 1./sqrt(gx*gx + gy*gy + 1);
-        nx = gx*nz;
-        ny = gy*nz;
-        return;
-    }
+        break;
+    case BT_POLYNOMIAL:                              // polynomialSurface.cpp:39-48
+        if constexpr (K == kKindsAll) {
+            double gx = -horner2d(x-s->p[0], y-s->p[1], s->a1, s->n1-1, s->n2-1);
+            double gy = -horner2d(x-s->p[0], y-s->p[1], s->a2, s->n1-1, s->n2-1);
+            nz = 1./sqrt(gx*gx + gy*gy + 1);
+            nx = gx*nz;
+            ny = gy*nz;
+            return;
+        }
+        break;
     }
     nx = __builtin_nan(""); ny = nx; nz = nx;
 }
 
 // Sum records are followed by their children (sum.cpp:23-45).
+template <int K = kKindsAll>
 BT_DEV double surf_sag(const bt_surface* s, double x, double y)
 {
-    if (s->kind != BT_SUM) return leaf_sag(s, x, y);
+    if (K != kKindsAll || s->kind != BT_SUM) return leaf_sag<K>(s, x, y);
     double result = 0.0;
     const int nc = s->nchild;
     for (int i = 0; i < nc; i++)
-        result += leaf_sag(s + 1 + i, x, y);
+        result += leaf_sag<K>(s + 1 + i, x, y);
     return result;
 }
 
+template <int K = kKindsAll>
 BT_DEV void surf_normal(const bt_surface* s, double x, double y,
                         double& nx, double& ny, double& nz)
 {
-    if (s->kind != BT_SUM) { leaf_normal(s, x, y, nx, ny, nz); return; }
+    if (K != kKindsAll || s->kind != BT_SUM) { leaf_normal<K>(s, x, y, nx, ny, nz); return; }
     double sx = 0.0, sy = 0.0;
     const int nc = s->nchild;
     for (int i = 0; i < nc; i++) {
         double tnx, tny, tnz;
-        leaf_normal(s + 1 + i, x, y, tnx, tny, tnz);
+        leaf_normal<K>(s + 1 + i, x, y, tnx, tny, tnz);
         sx += tnx/tnz;
         sy += tny/tnz;
     }
@@ -363,10 +384,11 @@ BT_DEV void surf_normal(const bt_surface* s, double x, double y,
 
 // Sag and normal at one point (the reference evaluates them separately; the
 // results here are bit-identical, with shared work evaluated once).
+template <int K = kKindsAll>
 BT_DEV void leaf_sag_normal(const bt_surface* s, double x, double y, double& sag,
                             double& nx, double& ny, double& nz)
 {
-    if (s->kind == BT_BICUBIC) {                     // bicubic.cpp:19-40
+    if (K == kKindsAll && s->kind == BT_BICUBIC) {                     // bicubic.cpp:19-40
         double dxdz, dydz;
         table_eval_grad(s, x, y, sag, dxdz, dydz);
         if (__builtin_isnan(dxdz)) {
@@ -378,19 +400,20 @@ BT_DEV void leaf_sag_normal(const bt_surface* s, double x, double y, double& sag
         ny = -dydz*nz;
         return;
     }
-    sag = leaf_sag(s, x, y);
-    leaf_normal(s, x, y, nx, ny, nz);
+    sag = leaf_sag<K>(s, x, y);
+    leaf_normal<K>(s, x, y, nx, ny, nz);
 }
 
+template <int K = kKindsAll>
 BT_DEV void surf_sag_normal(const bt_surface* s, double x, double y, double& sag,
                             double& nx, double& ny, double& nz)
 {
-    if (s->kind != BT_SUM) { leaf_sag_normal(s, x, y, sag, nx, ny, nz); return; }
+    if (K != kKindsAll || s->kind != BT_SUM) { leaf_sag_normal<K>(s, x, y, sag, nx, ny, nz); return; }
     double result = 0.0, sx = 0.0, sy = 0.0;
     const int nc = s->nchild;
     for (int i = 0; i < nc; i++) {
         double ts, tnx, tny, tnz;
-        leaf_sag_normal(s + 1 + i, x, y, ts, tnx, tny, tnz);
+        leaf_sag_normal<K>(s + 1 + i, x, y, ts, tnx, tny, tnz);
         result += ts;
         sx += tnx/tnz;
         sy += tny/tnz;
@@ -422,6 +445,7 @@ struct NormalMemo {
     bool ok;
 };
 
+template <int K = kKindsAll>
 BT_DEV bool newton(const bt_surface* s, double x, double y, double z,
                    double vx, double vy, double vz, double& dt, int niter,
                    NormalMemo* memo = nullptr)
@@ -429,11 +453,11 @@ BT_DEV bool newton(const bt_surface* s, double x, double y, double z,
     double rPx = x+vx*dt;
     double rPy = y+vy*dt;
     double rPz = z+vz*dt;
-    double sz = surf_sag(s, rPx, rPy);
+    double sz = surf_sag<K>(s, rPx, rPy);
     double pdt = 0.0;            // the iterate before the current one (iter >= 1)
     for (int iter = 0; iter < niter; iter++) {
         double nx, ny, nz;
-        surf_normal(s, rPx, rPy, nx, ny, nz);
+        surf_normal<K>(s, rPx, rPy, nx, ny, nz);
         if (memo) *memo = NormalMemo{dt, nx, ny, nz, true};
         double dtn = (rPx-x)*nx + (rPy-y)*ny + (sz-z)*nz;
         dtn /= (nx*vx + ny*vy + nz*vz);
@@ -443,7 +467,7 @@ BT_DEV bool newton(const bt_surface* s, double x, double y, double z,
         rPx = x+vx*dt;
         rPy = y+vy*dt;
         rPz = z+vz*dt;
-        sz = surf_sag(s, rPx, rPy);
+        sz = surf_sag<K>(s, rPx, rPy);
         if (iter >= 1 && same_bits(dt, pdt)) {
             // two-cycle: iterates alternate between this state and the
             // previous one; after the remaining (niter-1-iter) iterations the
@@ -486,6 +510,7 @@ BT_DEV bool quadric_tti(const bt_surface* s, double x, double y, double z,
     return true;
 }
 
+template <int K = kKindsAll>
 BT_DEV bool leaf_tti(const bt_surface* s, double x, double y, double z,
                      double vx, double vy, double vz, double& dt, int niter,
                      NormalMemo* memo = nullptr)
@@ -539,7 +564,7 @@ BT_DEV bool leaf_tti(const bt_surface* s, double x, double y, double z,
         return quadric_tti(s, x, y, z, vx, vy, vz, dt);
     case BT_ASPHERE:                                 // asphere.cpp:75-85
         if (!quadric_tti(s, x, y, z, vx, vy, vz, dt)) return false;
-        return newton(s, x, y, z, vx, vy, vz, dt, niter, memo);
+        return newton<K>(s, x, y, z, vx, vy, vz, dt, niter, memo);
     case BT_TILTED: {                                // tilted.cpp:29-37
         double num = z - x*s->p[0] - y*s->p[1];
         double den = vx*s->p[0] + vy*s->p[1] - vz;
@@ -549,28 +574,32 @@ BT_DEV bool leaf_tti(const bt_surface* s, double x, double y, double z,
     }
     case BT_BICUBIC:
     case BT_POLYNOMIAL:                              // surface.cpp:22-49
-        return newton(s, x, y, z, vx, vy, vz, dt, niter, memo);
+        if constexpr (K == kKindsAll) return newton<K>(s, x, y, z, vx, vy, vz, dt, niter, memo);
+        break;
     }
     return false;
 }
 
+template <int K = kKindsAll>
 BT_DEV bool surf_tti(const bt_surface* s, double x, double y, double z,
                      double vx, double vy, double vz, double& dt, int niter,
                      NormalMemo* memo = nullptr)
 {
-    if (s->kind != BT_SUM) return leaf_tti(s, x, y, z, vx, vy, vz, dt, niter, memo);
+    if (K != kKindsAll || s->kind != BT_SUM)
+        return leaf_tti<K>(s, x, y, z, vx, vy, vz, dt, niter, memo);
     // sum.cpp:47-56: first child's intersection seeds Newton on the Sum
-    if (!leaf_tti(s + 1, x, y, z, vx, vy, vz, dt, niter)) return false;
-    return newton(s, x, y, z, vx, vy, vz, dt, niter, memo);
+    if (!leaf_tti<K>(s + 1, x, y, z, vx, vy, vz, dt, niter)) return false;
+    return newton<K>(s, x, y, z, vx, vy, vz, dt, niter, memo);
 }
 
 // The surface's analytic base start: the root of a Sum's first child or of an
 // Asphere's conic (the starts of sum.cpp:53 and asphere.cpp:82, without the
 // Newton refinement that follows them); surfaces without one start at 0.
+template <int K = kKindsAll>
 BT_DEV bool conic_start(const bt_surface* s, double x, double y, double z, double vx,
                         double vy, double vz, double& dt)
 {
-    const bt_surface* c = (s->kind == BT_SUM) ? s + 1 : s;
+    const bt_surface* c = (K == kKindsAll && s->kind == BT_SUM) ? s + 1 : s;
     dt = 0.0;
     switch (c->kind) {
     case BT_ASPHERE:
@@ -580,7 +609,7 @@ BT_DEV bool conic_start(const bt_surface* s, double x, double y, double z, doubl
     case BT_SPHERE:
     case BT_PARABOLOID:
     case BT_TILTED:
-        return leaf_tti(c, x, y, z, vx, vy, vz, dt, 0);
+        return leaf_tti<K>(c, x, y, z, vx, vy, vz, dt, 0);
     default:
         return true;
     }
@@ -600,12 +629,13 @@ constexpr int kFastIter = 8;
 // 1e-13 m the residual position errors, carried through a three-mirror
 // optic, reach ~1e-14 in the final direction cosines; 1e-14 m keeps them an
 // order of magnitude below that.
+template <int K = kKindsAll>
 BT_DEV bool newton_fast(const bt_surface* s, double x, double y, double z,
                         double vx, double vy, double vz, double& dt, NormalMemo* memo)
 {
     double rPx = x+vx*dt, rPy = y+vy*dt, rPz = z+vz*dt;
     double sz, nx, ny, nz;
-    surf_sag_normal(s, rPx, rPy, sz, nx, ny, nz);
+    surf_sag_normal<K>(s, rPx, rPy, sz, nx, ny, nz);
     for (int it = 0; it < kFastIter; it++) {
         if (fabs(sz-rPz) <= kFastTol) break;
         double d = (rPx-x)*nx + (rPy-y)*ny + (sz-z)*nz;
@@ -613,7 +643,7 @@ BT_DEV bool newton_fast(const bt_surface* s, double x, double y, double z,
         rPx = x+vx*dt;
         rPy = y+vy*dt;
         rPz = z+vz*dt;
-        surf_sag_normal(s, rPx, rPy, sz, nx, ny, nz);
+        surf_sag_normal<K>(s, rPx, rPy, sz, nx, ny, nz);
     }
     if (!(fabs(sz-rPz) <= kFastTol)) return false;
     *memo = NormalMemo{dt, nx, ny, nz, true};
@@ -624,21 +654,22 @@ BT_DEV bool newton_fast(const bt_surface* s, double x, double y, double z,
 // ray origin where the surface has no conic base) plus `correction` (the
 // network's prediction for BT_START_MLP, 0 for BT_START_CONIC).  The
 // reference path decides every ray the fast start does not settle.
+template <int K = kKindsAll>
 BT_DEV bool fast_tti(const bt_surface* s, double x, double y, double z, double vx,
                      double vy, double vz, double& dt, int niter, double correction,
                      NormalMemo* memo)
 {
     if (newton_surface(s)) {
         double d0;
-        const bool ok = conic_start(s, x, y, z, vx, vy, vz, d0);
+        const bool ok = conic_start<K>(s, x, y, z, vx, vy, vz, d0);
         d0 += correction;
-        if (ok && d0 == d0 && newton_fast(s, x, y, z, vx, vy, vz, d0, memo)) {
+        if (ok && d0 == d0 && newton_fast<K>(s, x, y, z, vx, vy, vz, d0, memo)) {
             dt = d0;
             return true;
         }
     }
     dt = 0.0;
-    return surf_tti(s, x, y, z, vx, vy, vz, dt, niter, memo);
+    return surf_tti<K>(s, x, y, z, vx, vy, vz, dt, niter, memo);
 }
 
 // ---------------------------------------------------------------------------
@@ -788,6 +819,7 @@ struct Ray {
 // reference evaluates it after the intersection (batoid.cpp:474), but it is
 // a pure function of the wavelength, so callers may evaluate it once per ray
 // and medium and reuse it across Interfaces.
+template <int K = kKindsAll>
 BT_DEV bool apply_interaction(int action, const double* __restrict__ dr,
                               const double* __restrict__ drot,
                               const bt_surface* surf, double n2,
@@ -809,15 +841,15 @@ BT_DEV bool apply_interaction(int action, const double* __restrict__ dr,
     NormalMemo memo;
     memo.ok = false;
     const bool hit = (start == BT_START_REFERENCE)
-                         ? surf_tti(surf, x, y, z, vx, vy, vz, dt, niter, &memo)
-                         : fast_tti(surf, x, y, z, vx, vy, vz, dt, niter, correction, &memo);
+                         ? surf_tti<K>(surf, x, y, z, vx, vy, vz, dt, niter, &memo)
+                         : fast_tti<K>(surf, x, y, z, vx, vy, vz, dt, niter, correction, &memo);
     // the normal at the intersection, reusing Newton's when it was evaluated
     // at this very point
     auto normal_at = [&](double px, double py, double& nx, double& ny, double& nz) {
         if (memo.ok && same_bits(memo.dt, dt)) {
             nx = memo.nx; ny = memo.ny; nz = memo.nz;
         } else {
-            surf_normal(surf, px, py, nx, ny, nz);
+            surf_normal<K>(surf, px, py, nx, ny, nz);
         }
     };
     if (!hit) {

@@ -199,6 +199,7 @@ __device__ __forceinline__ double pick_medium(const double (&nm)[kPlanMedia], in
 // The network's input at an MLP step: the ray's point on the surface's base
 // conic and its direction, in the step's local frame (the frame change is
 // apply_interaction's).
+template <int K>
 __device__ __forceinline__ void mlp_features(const bt_step* __restrict__ st,
                                              const bt_surface* surf, const Ray& r,
                                              float (&f)[kMlpIn])
@@ -213,7 +214,7 @@ __device__ __forceinline__ void mlp_features(const bt_step* __restrict__ st,
     double vy = r.vx*m[1] + r.vy*m[4] + r.vz*m[7];
     double vz = r.vx*m[2] + r.vy*m[5] + r.vz*m[8];
     double dtc = 0.0;
-    if (r.fail || !conic_start(surf, x, y, z, vx, vy, vz, dtc)) {
+    if (r.fail || !conic_start<K>(surf, x, y, z, vx, vy, vz, dtc)) {
         // keep this lane's share of the wave's evaluation finite
         const MlpPacked* P = static_cast<const MlpPacked*>(st->mlp);
         for (int k = 0; k < kMlpIn; k++) f[k] = P->norm[k];
@@ -232,7 +233,7 @@ __device__ __forceinline__ void mlp_features(const bt_step* __restrict__ st,
 // the Newton-heavy optics), mode 2 to three (left alone the compiler would
 // settle for two); mode 1 keeps the compiler's three, where four measured
 // slower on the TMA.
-template <int Mode>
+template <int Mode, int K>
 __global__ __launch_bounds__(kBlock)
 __attribute__((amdgpu_waves_per_eu(Mode == 0 ? 4 : (Mode == 2 ? 3 : 1))))
 void k_plan(
@@ -281,10 +282,10 @@ void k_plan(
         if (start == BT_START_MLP && (!Mlp || !st->mlp)) start = BT_START_CONIC;
         if (Mlp && start == BT_START_MLP) {     // uniform: whole wave
             float f[kMlpIn];
-            mlp_features(st, surf, r, f);
+            mlp_features<K>(st, surf, r, f);
             correction = (double)mlp_predict_wave(static_cast<const MlpPacked*>(st->mlp), f);
         }
-        apply_interaction(action, st->dr, st->drot, surf, n2, r, niter, start, correction);
+        apply_interaction<K>(action, st->dr, st->drot, surf, n2, r, niter, start, correction);
         const int olen = st->obsc_len;
         if (olen > 0 && !r.vig)
             r.vig = obsc_contains(plan.obscs + st->obsc, olen, r.x, r.y);
@@ -507,15 +508,17 @@ int bt_trace_plan(bt_plan plan, bt_rays in, bt_rays out, int niter, void* stream
     if (niter < 0) return fail(BT_E_SIZE, "bt_trace_plan: negative niter");
     if (in.n == 0) return 0;
     const int need_w = (plan.flags & BT_PLAN_NO_REFRACT) ? 0 : 1;
-    if (plan.flags & BT_PLAN_MLP_START) {
-        hipLaunchKernelGGL(k_plan<2>, dim3(grid_for(in.n)), dim3(kBlock), 0,
-                           as_stream(stream), plan, in, out, niter, need_w);
-    } else if (plan.flags & BT_PLAN_FAST_START) {
-        hipLaunchKernelGGL(k_plan<1>, dim3(grid_for(in.n)), dim3(kBlock), 0,
-                           as_stream(stream), plan, in, out, niter, need_w);
-    } else {
-        hipLaunchKernelGGL(k_plan<0>, dim3(grid_for(in.n)), dim3(kBlock), 0,
-                           as_stream(stream), plan, in, out, niter, need_w);
+    const bool analytic = (plan.flags & BT_PLAN_ANALYTIC) != 0;
+    const int mode = (plan.flags & BT_PLAN_MLP_START) ? 2 : (plan.flags & BT_PLAN_FAST_START) ? 1 : 0;
+    const dim3 grid(grid_for(in.n)), block(kBlock);
+    hipStream_t st = as_stream(stream);
+    switch (mode * 2 + (analytic ? 1 : 0)) {
+    case 0: hipLaunchKernelGGL((k_plan<0, kKindsAll>), grid, block, 0, st, plan, in, out, niter, need_w); break;
+    case 1: hipLaunchKernelGGL((k_plan<0, kKindsAnalytic>), grid, block, 0, st, plan, in, out, niter, need_w); break;
+    case 2: hipLaunchKernelGGL((k_plan<1, kKindsAll>), grid, block, 0, st, plan, in, out, niter, need_w); break;
+    case 3: hipLaunchKernelGGL((k_plan<1, kKindsAnalytic>), grid, block, 0, st, plan, in, out, niter, need_w); break;
+    case 4: hipLaunchKernelGGL((k_plan<2, kKindsAll>), grid, block, 0, st, plan, in, out, niter, need_w); break;
+    default: hipLaunchKernelGGL((k_plan<2, kKindsAnalytic>), grid, block, 0, st, plan, in, out, niter, need_w); break;
     }
     return check_launch("bt_trace_plan");
 }

@@ -120,9 +120,11 @@ class PlanSpec:
             cs = iface.coordSys
         self.coordSys_in = coordSys_in
         self.coordSys_out = cs
+        analytic = all(r.kind in _lib.ANALYTIC_KINDS for r in self.surf_recs)
         self.flags = ((0 if refracts else _lib.PLAN_NO_REFRACT)
                       | (_lib.PLAN_FAST_START if fast else 0)
-                      | (_lib.PLAN_MLP_START if mlp else 0))
+                      | (_lib.PLAN_MLP_START if mlp else 0)
+                      | (_lib.PLAN_ANALYTIC if analytic else 0))
 
 
 class DevicePlan:

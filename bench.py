@@ -33,9 +33,8 @@ BYTES_PER_RAY = 8 * 8 + 2 + 7 * 8 + 2
 # HBM bytes per ray of the same kernel on this workload, measured with
 # rocprofv3 FETCH_SIZE and WRITE_SIZE (each scaled by its reading for a
 # 1 GiB device copy: x2.000 and x1.000), from
-# profiles/trace_plan_tma_1e7_counters.md: the algorithmic 124 B plus the
-# stores of the registers the kernel spills at four waves per SIMD.
-MEASURED_TRAFFIC_PER_RAY = 188.05
+# profiles/trace_plan_tma_1e7_counters.md.
+MEASURED_TRAFFIC_PER_RAY = 124.04
 # FP64 flops per ray of the same kernel on this workload: 64 lanes x
 # (ADD + MUL + 2 FMA) FP64 wave instructions (SQ_INSTS_VALU_*_F64), same file.
 FP64_FLOP_PER_RAY = 7024.0
@@ -472,7 +471,7 @@ def main():
             "niter": _batoid_niter, "unvignetted_fraction": round(frac_unvig, 4),
         },
         "roofline": {
-            "kernel": "k_plan (bt_trace_plan)",
+            "kernel": "k_plan<0, kKindsAnalytic> (bt_trace_plan)",
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,

@@ -184,6 +184,12 @@ typedef struct bt_step {
 #define BT_PLAN_FAST_START 2   /* some step uses a CONIC or MLP start      */
 #define BT_PLAN_MLP_START  4   /* some step uses an MLP start; without this
                                   flag MLP steps run as CONIC             */
+#define BT_PLAN_ANALYTIC   8   /* every surface record is a Plane, Sphere,
+                                  Paraboloid, Quadric, Asphere or Tilted:
+                                  the trace runs a kernel without the
+                                  Bicubic, polynomial and Sum code (fewer
+                                  registers); under this flag any other kind
+                                  evaluates to NaN and fails              */
 
 typedef struct bt_plan {
     int32_t nstep;

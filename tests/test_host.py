@@ -9,6 +9,7 @@ import pytest
 import batoid_amd as bt
 import oracle
 from oracle.host import host_records
+from batoid_amd import _lib
 from batoid_amd.plan import PlanSpec, collect_steps
 from batoid_amd.zernike import noll_to_nm, zernike_coef_arrays
 from batoid_amd import workloads
@@ -132,7 +133,13 @@ def test_yaml_and_plan():
         'Filter_entrance', 'Filter_exit', 'L3_entrance', 'L3_exit', 'Detector']
     assert [s[1] for s in steps] == [1] * 3 + [2] * 8 + [0]
     spec = PlanSpec(steps, bt.CoordSys())
-    assert len(spec.steps) == 12 and spec.flags == 0
+    # refracting, reference starts, only analytic and Asphere surfaces
+    assert len(spec.steps) == 12 and spec.flags == _lib.PLAN_ANALYTIC
+    assert PlanSpec(steps, bt.CoordSys(), "conic").flags == (
+        _lib.PLAN_ANALYTIC | _lib.PLAN_FAST_START)
+    # a Sum (or Bicubic / Zernike) surface needs the all-kinds kernel
+    pert = workloads.perturbed_tma()
+    assert PlanSpec(collect_steps(pert), bt.CoordSys()).flags == 0
     assert len(spec.med_recs) == 2                     # silica and vacuum, shared
     assert isinstance(optic['L2_exit'].surface, bt.Asphere)
     assert isinstance(optic['M1'].obscuration, bt.ObscNegation)
