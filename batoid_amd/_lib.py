@@ -77,7 +77,25 @@ class BtPsfSegment(ctypes.Structure):
     ]
 
 
-STRUCTS = (BtRays, BtSurface, BtMedium, BtObsc, BtStep, BtPlan, BtPsfSegment)
+class BtRaygen(ctypes.Structure):
+    _fields_ = [
+        ("pattern", ctypes.c_int32), ("nring", ctypes.c_int32),
+        ("nx", ctypes.c_int64), ("ny", ctypes.c_int64),
+        ("fx", ctypes.c_double), ("fy", ctypes.c_double),
+        ("lx", ctypes.c_double * 2), ("ly", ctypes.c_double * 2),
+        ("lo", ctypes.c_double * 2), ("span", ctypes.c_double * 2),
+        ("pcg_state", ctypes.c_uint64 * 2), ("pcg_inc", ctypes.c_uint64 * 2),
+        ("ring_off", ctypes.c_void_p), ("ring_rho", ctypes.c_void_p),
+        ("ring_dth", ctypes.c_void_p), ("px", ctypes.c_void_p), ("py", ctypes.c_void_p),
+        ("flux", ctypes.c_void_p), ("stop", ctypes.c_void_p),
+        ("stop_dr", ctypes.c_double * 3), ("stop_drot", ctypes.c_double * 9),
+        ("origin", ctypes.c_double * 3), ("rot", ctypes.c_double * 9),
+        ("vv", ctypes.c_double * 3), ("wavelength", ctypes.c_double),
+        ("flux0", ctypes.c_double),
+    ]
+
+
+STRUCTS = (BtRays, BtSurface, BtMedium, BtObsc, BtStep, BtPlan, BtPsfSegment, BtRaygen)
 
 # enum values (include/batoid_amd.h)
 PLANE, SPHERE, PARABOLOID, QUADRIC, ASPHERE, TILTED, BICUBIC, POLYNOMIAL, SUM = range(9)
@@ -92,6 +110,7 @@ PLAN_MLP_START = 4
 PLAN_ANALYTIC = 8
 ANALYTIC_KINDS = (PLANE, SPHERE, PARABOLOID, QUADRIC, ASPHERE, TILTED)
 PSF_FP32, PSF_BF16X3 = 0, 1
+GEN_POINTS, GEN_GRID, GEN_GRID_RANDOM, GEN_POLAR, GEN_POLAR_RANDOM = range(5)
 OBSC_MAX_STACK = 32
 MAX_SUM_CHILDREN = 8
 
@@ -110,6 +129,7 @@ SIGNATURES = {
     "bt_apply_forward_transform_arrays": (_I, [_V, _V, _V, _V, _V, _L, _V]),
     "bt_apply_reverse_transform_arrays": (_I, [_V, _V, _V, _V, _V, _L, _V]),
     "bt_finish_parallel": (_I, [_V, _V, _V, _V, _V, _V, _L, _V]),
+    "bt_generate_rays": (_I, [_V, BtRays, _V]),
     "bt_intersect": (_I, [_V, _V, _V, BtRays, _I, _V, _I, _V]),
     "bt_reflect": (_I, [_V, _V, _V, BtRays, _I, _V, _I, _V]),
     "bt_refract": (_I, [_V, _V, _V, _V, BtRays, _I, _V, _I, _V]),

@@ -375,12 +375,12 @@ const char* bt_last_error(void) { return g_last_error.c_str(); }
 
 int bt_struct_sizes(int64_t* sizes, int n)
 {
-    const int64_t all[7] = {(int64_t)sizeof(bt_rays), (int64_t)sizeof(bt_surface),
+    const int64_t all[8] = {(int64_t)sizeof(bt_rays), (int64_t)sizeof(bt_surface),
                             (int64_t)sizeof(bt_medium), (int64_t)sizeof(bt_obsc),
                             (int64_t)sizeof(bt_step), (int64_t)sizeof(bt_plan),
-                            (int64_t)sizeof(bt_psf_segment)};
+                            (int64_t)sizeof(bt_psf_segment), (int64_t)sizeof(bt_raygen)};
     if (!sizes) return fail(BT_E_NULL, "bt_struct_sizes: null argument");
-    for (int i = 0; i < n && i < 7; i++) sizes[i] = all[i];
+    for (int i = 0; i < n && i < 8; i++) sizes[i] = all[i];
     return 0;
 }
 

@@ -8,15 +8,16 @@ counterpart of the reference's syncToHost on property access
 (rayVector.py:1083-1176); ``rv.torch(name)`` gives the device column
 itself without a copy.
 """
+import ctypes
 from numbers import Real
 
 import numpy as np
 import torch
 
-from . import _lib, _device
+from . import _lib, _device, _pcg
 from .constants import globalCoordSys, vacuum
 from .coordTransform import CoordTransform
-from .utils import fieldToDirCos, hexapolar
+from .utils import fieldToDirCos
 
 _FCOLS = ("x", "y", "z", "vx", "vy", "vz", "t", "wavelength", "flux")
 _BCOLS = ("vignetted", "failed")
@@ -240,26 +241,18 @@ class RayVector:
         return backDist, medium, stopSurface, coordSys
 
     @classmethod
-    def _onStop(cls, x, y, stopSurface, coordSys, wavelength, medium, backDist,
-                source, dirCos, flux):
-        """Place rays on the stop surface at (x, y), move them to ``coordSys``
-        and back them up to the entrance plane (rayVector.py:438-444)."""
-        from .trace import applyForwardTransformArrays
-        x = np.ascontiguousarray(x, dtype=float)
-        y = np.ascontiguousarray(y, dtype=float)
-        z = stopSurface.surface.sag(x, y)
-        xd, yd, zd = _device.f64(x), _device.f64(y), _device.f64(np.atleast_1d(z))
-        transform = CoordTransform(stopSurface.coordSys, coordSys)
-        applyForwardTransformArrays(transform, xd, yd, zd)
-        n = medium.getN(wavelength)
-        return cls._finish(backDist, source, dirCos, n, xd, yd, zd, wavelength, flux, coordSys)
-
-    @classmethod
-    def _finish(cls, backDist, source, dirCos, n, x, y, z, wavelength, flux, coordSys):
-        """Map rays back to their source plane (rayVector.py:816-858)."""
+    def _generate(cls, gen, npts, stopSurface, coordSys, wavelength, medium,
+                  backDist, source, dirCos, flux, out=None):
+        """Write ``npts`` new rays on the GPU: stop coordinates from ``gen``
+        (a BtRaygen with its pattern fields set), the stop surface's sag,
+        the stop -> ``coordSys`` transform (rayVector.py:435-437), then
+        finishParallel back to the source plane and the constant columns
+        (rayVector.py:816-841), in one bt_generate_rays pass.  ``out``: an
+        existing RayVector of ``npts`` rays to overwrite instead of new
+        columns."""
         if source is not None:
             raise NotImplementedError("rays from a finite source point are not supported")
-        npts = x.numel()
+        n = medium.getN(wavelength)
         vv = np.array(dirCos, dtype=float, copy=True)
         vv /= n * np.sqrt(np.dot(vv, vv))
         zhat = -n * vv
@@ -268,26 +261,59 @@ class RayVector:
         yhat = np.cross(xhat, zhat)
         origin = zhat * backDist
         rot = np.stack([xhat, yhat, zhat]).T
-        _lib.call("bt_finish_parallel", _lib.dvec(origin), _lib.dvec(rot.ravel()),
-                  _lib.dvec(vv), _device.ptr(x), _device.ptr(y), _device.ptr(z), npts,
-                  _device.stream())
+        transform = CoordTransform(stopSurface.coordSys, coordSys)
+        gen.stop = stopSurface.surface._dev.ptr()
+        gen.stop_dr[:] = [float(v) for v in transform.dr]
+        gen.stop_drot[:] = [float(v) for v in np.asarray(transform.drot).ravel()]
+        gen.origin[:] = [float(v) for v in origin]
+        gen.rot[:] = [float(v) for v in rot.ravel()]
+        gen.vv[:] = [float(v) for v in vv]
+        gen.wavelength = float(wavelength)
         dev = _device.device()
+        fluxd = None
         if isinstance(flux, Real):
-            fluxd = torch.full((npts,), float(flux), dtype=torch.float64, device=dev)
+            gen.flux0 = float(flux)
         else:
             fluxd = _device.f64(np.broadcast_to(np.asarray(flux, float), (npts,)).ravel())
-        cols = {
-            "x": x, "y": y, "z": z,
-            "vx": torch.full((npts,), vv[0], dtype=torch.float64, device=dev),
-            "vy": torch.full((npts,), vv[1], dtype=torch.float64, device=dev),
-            "vz": torch.full((npts,), vv[2], dtype=torch.float64, device=dev),
-            "t": torch.zeros(npts, dtype=torch.float64, device=dev),
-            "wavelength": torch.full((npts,), float(wavelength), dtype=torch.float64, device=dev),
-            "flux": fluxd,
-            "vignetted": torch.zeros(npts, dtype=torch.uint8, device=dev),
-            "failed": torch.zeros(npts, dtype=torch.uint8, device=dev),
-        }
-        return cls._directInit(cols, coordSys)
+            gen.flux = fluxd.data_ptr()
+        if out is not None:
+            if len(out) != npts:
+                raise ValueError(f"out holds {len(out)} rays, the generator makes {npts}")
+            rv = out
+            rv.coordSys = coordSys
+        else:
+            cols = {k: torch.empty(npts, dtype=torch.float64, device=dev) for k in _FCOLS}
+            for k in _BCOLS:
+                cols[k] = torch.empty(npts, dtype=torch.uint8, device=dev)
+            rv = cls._directInit(cols, coordSys)
+        # inputs (tables, points, fluxes) are torch tensors on the current
+        # stream: their memory is not reused before this launch has read it
+        _lib.call("bt_generate_rays", ctypes.byref(gen), rv._rays, _device.stream())
+        del fluxd
+        return rv
+
+    @staticmethod
+    def _random_pattern(gen, rng, nrandom, lo, span):
+        """Point ``gen`` at the PCG64 stream of ``rng`` for two blocks of
+        ``nrandom`` uniforms; returns False for other bit generators."""
+        words = _pcg.stream_words(rng)
+        if words is None:
+            return False
+        (gen.pcg_state[0], gen.pcg_state[1]), (gen.pcg_inc[0], gen.pcg_inc[1]) = words
+        gen.lo[:] = [float(v) for v in lo]
+        gen.span[:] = [float(v) for v in span]
+        _pcg.consume(rng, 2 * nrandom)
+        return True
+
+    @classmethod
+    def _points(cls, x, y, *args):
+        x = np.ascontiguousarray(np.atleast_1d(x), dtype=float)
+        y = np.ascontiguousarray(np.atleast_1d(y), dtype=float)
+        gen = _lib.BtRaygen()
+        gen.pattern = _lib.GEN_POINTS
+        xd, yd = _device.f64(x), _device.f64(y)
+        gen.px, gen.py = xd.data_ptr(), yd.data_ptr()
+        return cls._generate(gen, x.size, *args)
 
     @classmethod
     def asGrid(cls, optic=None, backDist=None, medium=None, stopSurface=None,
@@ -324,26 +350,32 @@ class RayVector:
             lx = (lx, 0.0)
         if isinstance(ly, Real):
             ly = (0.0, ly)
+        args = (stopSurface, coordSys, wavelength, medium, backDist, source, dirCos, flux)
+        gen = _lib.BtRaygen()
+        gen.lx[:] = [float(v) for v in lx]
+        gen.ly[:] = [float(v) for v in ly]
         if nrandom is not None:
             if rng is None:
                 rng = np.random.default_rng()
             elif isinstance(rng, int):
                 rng = np.random.default_rng(rng)
+            nrandom = int(nrandom)
+            if cls._random_pattern(gen, rng, nrandom, (-0.5, -0.5), (0.5 - -0.5,) * 2):
+                gen.pattern = _lib.GEN_GRID_RANDOM
+                return cls._generate(gen, nrandom, *args)
+            # another bit generator: its draws come from numpy
             xx = rng.uniform(-0.5, 0.5, size=nrandom)
             yy = rng.uniform(-0.5, 0.5, size=nrandom)
-        else:
-            x_d = 1. if nx <= 2 else (nx - (2 if nx % 2 == 0 else 1)) / nx
-            y_d = 1. if ny <= 2 else (ny - (2 if ny % 2 == 0 else 1)) / ny
-            xx = np.fft.fftshift(np.fft.fftfreq(nx, x_d))
-            yy = np.fft.fftshift(np.fft.fftfreq(ny, y_d))
-            xx, yy = np.meshgrid(xx, yy)
-            xx = xx.ravel()
-            yy = yy.ravel()
-        stack = np.stack([xx, yy])
-        x = np.dot(lx, stack)
-        y = np.dot(ly, stack)
-        return cls._onStop(x, y, stopSurface, coordSys, wavelength, medium, backDist,
-                           source, dirCos, flux)
+            stack = np.stack([xx, yy])
+            return cls._points(np.dot(lx, stack), np.dot(ly, stack), *args)
+        # fftshift(fftfreq(n, d)) = (k - n//2) / (n d), k = 0..n-1
+        x_d = 1. if nx <= 2 else (nx - (2 if nx % 2 == 0 else 1)) / nx
+        y_d = 1. if ny <= 2 else (ny - (2 if ny % 2 == 0 else 1)) / ny
+        gen.pattern = _lib.GEN_GRID
+        gen.nx, gen.ny = int(nx), int(ny)
+        gen.fx = 1.0 / (nx * x_d)
+        gen.fy = 1.0 / (ny * y_d)
+        return cls._generate(gen, int(nx) * int(ny), *args)
 
     @classmethod
     def asPolar(cls, optic=None, backDist=None, medium=None, stopSurface=None,
@@ -367,20 +399,48 @@ class RayVector:
             dirCos = fieldToDirCos(theta_x, theta_y, projection=projection)
         if wavelength is None:
             raise ValueError("Missing wavelength keyword")
+        args = (stopSurface, coordSys, wavelength, medium, backDist, source, dirCos, flux)
+        gen = _lib.BtRaygen()
         if nrandom is None:
-            rr, th = hexapolar(outer=outer, inner=inner, nrad=nrad, naz=naz,
-                               kfold=kfold, rth=True)
-        else:
-            if rng is None:
-                rng = np.random.default_rng()
-            elif isinstance(rng, int):
-                rng = np.random.default_rng(rng)
-            rr = np.sqrt(rng.uniform(inner**2, outer**2, size=nrandom))
-            th = rng.uniform(0, 2 * np.pi, size=nrandom)
-        x = rr * np.cos(th)
-        y = rr * np.sin(th)
-        return cls._onStop(x, y, stopSurface, coordSys, wavelength, medium, backDist,
-                           source, dirCos, flux)
+            npts, _tables = cls._polar_tables(gen, outer, inner, nrad, naz, kfold)
+            return cls._generate(gen, npts, *args)
+        if rng is None:
+            rng = np.random.default_rng()
+        elif isinstance(rng, int):
+            rng = np.random.default_rng(rng)
+        nrandom = int(nrandom)
+        lo0, hi0 = inner**2, outer**2
+        if cls._random_pattern(gen, rng, nrandom, (lo0, 0.0), (hi0 - lo0, 2 * np.pi - 0.0)):
+            gen.pattern = _lib.GEN_POLAR_RANDOM
+            return cls._generate(gen, nrandom, *args)
+        rr = np.sqrt(rng.uniform(inner**2, outer**2, size=nrandom))
+        th = rng.uniform(0, 2 * np.pi, size=nrandom)
+        return cls._points(rr * np.cos(th), rr * np.sin(th), *args)
+
+    @staticmethod
+    def _polar_tables(gen, outer, inner, nrad, naz, kfold=6):
+        """Hexapolar rings (utils.py:512-531) as the device ring table of a
+        POLAR generator: ray j of ring k sits at th = j * (2 pi / nphi_k),
+        numpy's linspace(0, 2 pi, nphi_k, endpoint=False).  Returns the ray
+        count and the device tables (keep them alive until the launch)."""
+        rhos = np.linspace(outer, inner, nrad)
+        nphis = []
+        for rho in rhos:
+            nphi = int((naz * rho / outer) // kfold) * kfold
+            nphis.append(nphi if nphi != 0 else kfold)
+        if inner == 0.0:
+            nphis[-1] = 1
+            rhos[-1] = 0.0
+        off = np.zeros(len(nphis) + 1, dtype=np.int64)
+        off[1:] = np.cumsum(nphis)
+        dth = np.array([2 * np.pi / k for k in nphis])
+        off_d = torch.from_numpy(off).to(_device.device())
+        rho_d, dth_d = _device.f64(rhos), _device.f64(dth)
+        gen.pattern = _lib.GEN_POLAR
+        gen.nring = len(nphis)
+        gen.ring_off, gen.ring_rho, gen.ring_dth = (
+            off_d.data_ptr(), rho_d.data_ptr(), dth_d.data_ptr())
+        return int(off[-1]), (off_d, rho_d, dth_d)
 
     @classmethod
     def asSpokes(cls, optic=None, backDist=None, medium=None, stopSurface=None,
@@ -416,10 +476,8 @@ class RayVector:
         flux = np.broadcast_to(flux, rings.shape).ravel()
         rings = rings.ravel()
         spokes = spokes.ravel()
-        x = rings * np.cos(spokes)
-        y = rings * np.sin(spokes)
-        return cls._onStop(x, y, stopSurface, coordSys, wavelength, medium, backDist,
-                           source, dirCos, flux)
+        return cls._points(rings * np.cos(spokes), rings * np.sin(spokes), stopSurface,
+                           coordSys, wavelength, medium, backDist, source, dirCos, flux)
 
     @classmethod
     def fromStop(cls, x, y, optic=None, backDist=None, medium=None,
@@ -434,9 +492,7 @@ class RayVector:
             dirCos = fieldToDirCos(theta_x, theta_y, projection=projection)
         if wavelength is None:
             raise ValueError("Missing wavelength keyword")
-        x = np.atleast_1d(x).astype(float, copy=False)
-        y = np.atleast_1d(y).astype(float, copy=False)
-        return cls._onStop(x, y, stopSurface, coordSys, wavelength, medium, backDist,
+        return cls._points(x, y, stopSurface, coordSys, wavelength, medium, backDist,
                            source, dirCos, flux)
 
     @classmethod

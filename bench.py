@@ -38,6 +38,8 @@ MEASURED_TRAFFIC_PER_RAY = 124.04
 # FP64 flops per ray of the same kernel on this workload: 64 lanes x
 # (ADD + MUL + 2 FMA) FP64 wave instructions (SQ_INSTS_VALU_*_F64), same file.
 FP64_FLOP_PER_RAY = 7024.0
+# Ray generation writes 9 FP64 columns and 2 flag bytes per ray.
+RAYGEN_BYTES_PER_RAY = 9 * 8 + 2
 
 
 def log(*a):
@@ -95,8 +97,8 @@ def cpu_baseline(optic, rv, target_s=10.0):
                       f"with {threads} OpenMP threads"}
 
 
-def bench_huygens(optic, reps=10):
-    """C4: the TMA pupil on a 512 x 512 grid (one field, 620 nm), traced, then
+def bench_huygens(optic, reps=10, nx=512):
+    """C4: the TMA pupil on an nx x nx grid (one field, 620 nm), traced, then
     a 256 x 256 Huygens PSF at 0.5 um pixels by the MFMA GEMM (FP32 and
     split-BF16) and by the FP64 direct sum.  TFLOP/s counts 8 flops per
     complex multiply-add: 8 * pixels * (unvignetted rays)."""
@@ -105,10 +107,10 @@ def bench_huygens(optic, reps=10):
     from batoid_amd import _lib, _device, workloads
     thx, thy = workloads.tma_fields()
     dirCos = bt.utils.fieldToDirCos(thx[0], thy[0])
-    rays = bt.RayVector.asGrid(optic=optic, wavelength=620e-9, dirCos=dirCos, nx=512)
+    rays = bt.RayVector.asGrid(optic=optic, wavelength=620e-9, dirCos=dirCos, nx=nx)
     optic.trace(rays)
     nvalid = int((~rays.vignetted & ~rays.failed).sum())
-    cr = (512 // 2) * 512 + 512 // 2
+    cr = (nx // 2) * nx + nx // 2
     point = rays.r[cr]
     time_ = float(rays.t[0])
     npix = 256
@@ -156,6 +158,106 @@ def bench_huygens(optic, reps=10):
     res["bf16x3"]["frac_of_bf16_mfma_peak"] = 3 * res["bf16x3"]["tflops"] / BF16_MFMA_PEAK_TFLOPS
     res["speedup_fp32_vs_fp64_direct"] = res["direct"]["ms_per_psf"] / res["fp32"]["ms_per_psf"]
     res["speedup_bf16x3_vs_fp64_direct"] = res["direct"]["ms_per_psf"] / res["bf16x3"]["ms_per_psf"]
+    return res
+
+
+def bench_raygen(optic, reps=20, cpu=True):
+    """Ray generation: the C2 pupil, asPolar(outer=4.18, inner=2.558,
+    nrad=1120, naz=11080) at one field, written by one bt_generate_rays
+    pass (hexapolar ring lookup, sin/cos, stop sag and frame change,
+    finishParallel, constant columns).  HBM bound: 74 B written per ray.
+    Beside it, the reference's host path (numpy hexapolar + cos/sin, then the
+    serial sag / transform / finishParallel loops, here the oracle's C
+    restatement of them, one thread) on the same pupil."""
+    import torch
+    import batoid_amd as bt
+    from batoid_amd import _lib, _device, workloads
+    from batoid_amd.rayVector import RayVector
+    thx, thy = workloads.tma_fields()
+    wl = 630e-9
+    dirCos = bt.utils.fieldToDirCos(thx[0], thy[0])
+    # end to end through the public generator (host ring table + launch)
+    e2e = []
+    for _ in range(3):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        rv = workloads.tma_rays(optic, thx[0], thy[0], wl)
+        torch.cuda.synchronize()
+        e2e.append(time.perf_counter() - t0)
+        if len(e2e) < 3:
+            del rv
+    n = len(rv)
+    gen = _lib.BtRaygen()
+    npts, tables = RayVector._polar_tables(gen, 4.18, 2.558, 1120, 11080)
+    RayVector._generate(gen, npts, optic.stopSurface, optic.coordSys, wl, optic.inMedium,
+                        optic.backDist, None, dirCos, 1.0, out=rv)
+    lib = _lib.load()
+    stream = _device.stream()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        lib.bt_generate_rays(ctypes.byref(gen), rv._rays, stream)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    nbytes = RAYGEN_BYTES_PER_RAY * n
+    gbs = nbytes / (ms * 1e-3) / 1e9
+    res = {"workload": "C2 pupil asPolar(outer=4.18, inner=2.558, nrad=1120, naz=11080), "
+                       "field 0, 630 nm",
+           "rays": n, "kernel": "k_generate (bt_generate_rays)", "ms_per_generation": ms,
+           "rays_per_s": n / (ms * 1e-3), "end_to_end_ms": min(e2e) * 1e3,
+           "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": gbs / HBM_PEAK_GBS, "bytes_per_ray": RAYGEN_BYTES_PER_RAY}}
+    log(f"raygen: {ms:.3f} ms per {n} rays ({gbs:.0f} GB/s, {gbs / HBM_PEAK_GBS:.1%} of HBM), "
+        f"end to end {min(e2e) * 1e3:.1f} ms")
+    # the same pass with the lattice pattern (no ring search, no sin/cos):
+    # the write-stream rate the hexapolar pass is compared with
+    ng = 3162
+    grid = RayVector.asGrid(optic=optic, wavelength=wl, dirCos=dirCos, nx=ng,
+                            lx=optic.pupilSize)
+    ggen = _lib.BtRaygen()
+    ggen.pattern = _lib.GEN_GRID
+    ggen.nx = ggen.ny = ng
+    ggen.fx = ggen.fy = 1.0 / (ng * ((ng - 2) / ng))
+    ggen.lx[:] = [optic.pupilSize, 0.0]
+    ggen.ly[:] = [0.0, optic.pupilSize]
+    RayVector._generate(ggen, ng * ng, optic.stopSurface, optic.coordSys, wl, optic.inMedium,
+                        optic.backDist, None, dirCos, 1.0, out=grid)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(reps):
+        lib.bt_generate_rays(ctypes.byref(ggen), grid._rays, stream)
+    e1.record()
+    torch.cuda.synchronize()
+    gms = e0.elapsed_time(e1) / reps
+    res["lattice_same_size"] = {"rays": ng * ng, "ms_per_generation": gms,
+                                "GBps": RAYGEN_BYTES_PER_RAY * ng * ng / (gms * 1e-3) / 1e9}
+    log(f"raygen lattice {ng}x{ng}: {gms:.3f} ms "
+        f"({res['lattice_same_size']['GBps']:.0f} GB/s)")
+    del grid
+    if cpu:
+        import oracle
+        from oracle import raygen as G
+        from oracle.host import host_records
+        tab = host_records(optic.inMedium)
+        w = np.array([wl])
+        nidx = np.empty(1)
+        oracle.load().or_get_n_n(tab.ptr(), w.ctypes.data, nidx.ctypes.data, 1, 1)
+        ct = bt.CoordTransform(optic.stopSurface.coordSys, optic.coordSys)
+        t0 = time.perf_counter()
+        rr, th = G.hexapolar(4.18, 2.558, 1120, 11080)
+        x, y = G.polar_xy(rr, th)
+        G.finish(x, y, optic.stopSurface.surface, (ct.dr, ct.drot), dirCos, nidx[0],
+                 optic.backDist, wl)
+        t = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": n / t, "unit": "rays/s", "cores": 1, "kind": "port",
+                               "sample": f"the whole {n}-ray pupil, {t:.2f} s: numpy as the "
+                                         "reference + the C oracle's sag, transform and "
+                                         "finishParallel loops"}
+        log(f"raygen cpu: {t:.2f} s ({n / t:.3e} rays/s)")
+    del tables
     return res
 
 
@@ -331,6 +433,7 @@ def main():
     ap.add_argument("--no-huygens", action="store_true")
     ap.add_argument("--no-newton-start", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-raygen", action="store_true")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="also run a plain device copy of this many MiB (calibrates "
                          "FETCH_SIZE / WRITE_SIZE counters against known bytes)")
@@ -498,6 +601,10 @@ def main():
                 "converged to a 1e-14 m sag gap (not bit-identical to the reference)"}
     if not args.no_huygens:
         result["huygens"] = bench_huygens(optic)
+        # the same PSF from a ~2e5-ray pupil (asGrid nx=640 over the annulus)
+        result["huygens_nx640"] = bench_huygens(optic, nx=640)
+    if not args.no_raygen:
+        result["raygen"] = bench_raygen(optic, cpu=not args.no_cpu)
     if not args.no_newton_start:
         result["newton_start"] = {
             name: bench_newton_start(args.nrad, args.naz, name)

@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define BT_ABI_VERSION 2
+#define BT_ABI_VERSION 3
 
 /* ---- status codes ------------------------------------------------------ */
 #define BT_OK            0
@@ -206,7 +206,7 @@ typedef struct bt_plan {
 int bt_abi_version(void);
 const char* bt_last_error(void);
 /* sizeof of bt_rays, bt_surface, bt_medium, bt_obsc, bt_step, bt_plan,
- * bt_psf_segment, in that order, written to sizes[0..n-1]; lets a binding
+ * bt_psf_segment, bt_raygen, in that order, written to sizes[0..n-1]; lets a binding
  * check its layout. */
 int bt_struct_sizes(int64_t* sizes, int n);
 /* Device properties the host uses for launch sizing. */
@@ -233,6 +233,60 @@ int bt_apply_reverse_transform_arrays(const double* dr, const double* drot,
 int bt_finish_parallel(const double* dr, const double* drot, const double* vv,
                        double* x, double* y, double* z, int64_t n,
                        void* stream);
+
+/* ---- ray generation -----------------------------------------------------
+ * Replaces the body of RayVector.asGrid / asPolar / fromStop
+ * (batoid/rayVector.py:219-444, 459-643, 860-982) for a parallel beam: stop
+ * coordinates, the stop surface's sag, the stop -> ray frame transform
+ * (src/batoid.cpp:8-22), finishParallel (src/batoid.cpp:40-69) and the
+ * constant columns (rayVector.py:831-841), all in one pass that writes the
+ * 11 columns of `out` (out.n rays).
+ *   POINTS         stop (x, y) = (px[i], py[i])                 fromStop
+ *   GRID           ray i = (iy, ix) = (i / nx, i % nx) of the
+ *                  fftshift(fftfreq) lattice: u = (ix - nx/2) fx,
+ *                  w = (iy - ny/2) fy; (x, y) = (lx.(u, w), ly.(u, w))
+ *                                                 rayVector.py:418-433
+ *   GRID_RANDOM    u = lo[0] + span[0] U_i, w = lo[1] + span[1] U_(n+i)
+ *                                                 rayVector.py:410-416
+ *   POLAR          ring k holds rays [ring_off[k], ring_off[k+1]); ray j of
+ *                  the ring at r = ring_rho[k], th = j ring_dth[k]
+ *                                                 utils.py:512-531
+ *   POLAR_RANDOM   r = sqrt(lo[0] + span[0] U_i), th = lo[1] + span[1] U_(n+i)
+ *                                                 rayVector.py:625-630
+ * U_k is the k-th double (k from 0) a PCG64 (XSL-RR 128/64) generator with
+ * state pcg_state / increment pcg_inc yields, (next64 >> 11) * 2^-53: the
+ * stream numpy.random.Generator(PCG64).uniform draws, so a seeded
+ * default_rng gives the same rays as the reference.  POLAR* take
+ * (r cos th, r sin th) with the device's FP64 sin/cos.
+ * Every pointer is a device pointer; `flux` may be NULL (all rays flux0). */
+enum bt_gen_pattern {
+    BT_GEN_POINTS = 0, BT_GEN_GRID = 1, BT_GEN_GRID_RANDOM = 2,
+    BT_GEN_POLAR = 3, BT_GEN_POLAR_RANDOM = 4
+};
+
+typedef struct bt_raygen {
+    int32_t pattern;
+    int32_t nring;              /* POLAR: rings in the ring tables          */
+    int64_t nx, ny;             /* GRID: lattice shape, nx * ny == out.n    */
+    double fx, fy;              /* GRID: fftfreq spacing 1 / (n d)          */
+    double lx[2], ly[2];        /* GRID*: lattice edge vectors              */
+    double lo[2], span[2];      /* *_RANDOM: uniform(lo, lo + span)          */
+    uint64_t pcg_state[2];      /* *_RANDOM: PCG64 state {hi, lo}           */
+    uint64_t pcg_inc[2];        /*           increment {hi, lo}             */
+    const int64_t* ring_off;    /* POLAR: nring + 1 ray offsets             */
+    const double* ring_rho;     /* POLAR: nring radii                        */
+    const double* ring_dth;     /* POLAR: nring azimuth steps                */
+    const double* px;           /* POINTS: stop coordinates                  */
+    const double* py;
+    const double* flux;         /* NULL or out.n per-ray fluxes              */
+    const bt_surface* stop;     /* stop surface record                       */
+    double stop_dr[3], stop_drot[9];    /* stop frame -> ray frame           */
+    double origin[3], rot[9];   /* finishParallel plane (rayVector.py:822-829) */
+    double vv[3];               /* ray velocity (rayVector.py:820-821)       */
+    double wavelength, flux0;
+} bt_raygen;
+
+int bt_generate_rays(const bt_raygen* gen, bt_rays out, void* stream);
 
 /* ---- per-surface trace --------------------------------------------------
  * Replace _batoid.intersect / reflect / refract (src/batoid.cpp:175-506,

@@ -23,7 +23,7 @@ def test_header_declares_the_boundary():
     names = declared_functions()
     for required in ("bt_intersect", "bt_reflect", "bt_refract", "bt_obscure",
                      "bt_trace_plan", "bt_sum_amplitude", "bt_huygens_gemm",
-                     "bt_huygens_direct", "bt_finish_parallel"):
+                     "bt_huygens_direct", "bt_finish_parallel", "bt_generate_rays"):
         assert required in names
 
 
@@ -42,4 +42,4 @@ def test_struct_layouts_match():
     sizes = (ctypes.c_int64 * len(_lib.STRUCTS))()
     assert lib.bt_struct_sizes(sizes, len(_lib.STRUCTS)) == 0
     assert list(sizes) == [ctypes.sizeof(s) for s in _lib.STRUCTS]
-    assert lib.bt_abi_version() == 2
+    assert lib.bt_abi_version() == 3
