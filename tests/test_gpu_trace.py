@@ -117,6 +117,28 @@ def test_obscurations_bit_exact(oname, obsc):
     np.testing.assert_array_equal(obsc.contains(x, y), oc.astype(bool))
 
 
+def test_circular_edges_bit_exact():
+    """Points on the edge of circular obscurations (a hexapolar ring at the
+    obscuration radius, as a pupil's outermost ring meets a mirror's clear
+    aperture): contains() is decided by the last bit of hypot, and the
+    GPU's ref_hypot reproduces libm's, so the flags match the oracle's
+    (which calls libm) point for point."""
+    O = oracle.load()
+    th = np.arange(200_000) * (2 * np.pi / 200_000)
+    for obsc, r in [(bt.ObscCircle(4.1), 4.1), (bt.ObscNegation(bt.ObscCircle(4.1)), 4.1),
+                    (bt.ObscAnnulus(2.558, 4.18), 4.18), (bt.ObscAnnulus(2.558, 4.18), 2.558),
+                    (bt.ObscCircle(0.5, 0.1, -0.2), 0.5)]:
+        x0, y0 = (0.1, -0.2) if r == 0.5 else (0.0, 0.0)
+        x, y = x0 + r * np.cos(th), y0 + r * np.sin(th)
+        tab = host_records(obsc)
+        oc = np.empty(len(x), np.uint8)
+        O.or_contains_n(tab.ptr(), tab.count, x.ctypes.data, y.ctypes.data, oc.ctypes.data,
+                        len(x), nthreads())
+        got = obsc.contains(x, y)
+        assert 0 < oc.sum() < len(x)       # the ring really straddles the edge
+        np.testing.assert_array_equal(got, oc.astype(bool), err_msg=repr(obsc))
+
+
 @pytest.mark.parametrize("name,surf", surfaces(), ids=[s[0] for s in surfaces()])
 def test_sag_normal_bit_exact(name, surf):
     O = oracle.load()

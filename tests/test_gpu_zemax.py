@@ -59,3 +59,55 @@ def test_lsst_wavefront_vs_zemax():
                            reference='chief', projection='zemax')
         Zwf = np.ma.MaskedArray(data=Zwf, mask=Zwf == 0)
         np.testing.assert_allclose(Zwf * 500e-9, bwf.array * 500e-9, atol=1e-11, rtol=0)
+
+
+class GpuTracer:
+    """The GPU library through batoid's API (Optic.traceFull)."""
+
+    def rays(self, x, y, z, vx, vy, vz, t, w):
+        return bt.RayVector(x, y, z, vx, vy, vz, t=t, wavelength=w)
+
+    def from_stop(self, x, y, optic, stop, w, dirCos):
+        return bt.RayVector.fromStop(x, y, optic=optic, stopSurface=stop, wavelength=w,
+                                     dirCos=dirCos)
+
+    def propagate(self, rv, t):
+        rv.propagate(t)
+
+    def trace_full(self, optic, rv):
+        out = []
+        for name, d in optic.traceFull(rv).items():
+            o = d['out'].toCoordSys(bt.globalCoordSys)
+            out.append((name, o.r[0], o.v[0]))
+        return out
+
+
+@pytest.mark.parametrize("case", ["hsc", "decam", "cbp", "comcam"])
+def test_zemax_traces(case):
+    """HSC, DECam, CBP and ComCam Zemax traces (tests/test_zemax.py:15-667)."""
+    import zemax_cases
+    getattr(zemax_cases, case)(GpuTracer())
+
+
+@pytest.mark.parametrize("name", ["HSC_old.yaml", "DECam.yaml", "ComCam_r.yaml"])
+def test_fused_plan_bit_exact_on_golden_optics(name):
+    """The fused plan kernel on the golden optics (Baffles, tilted frames,
+    table/Sumita/Sellmeier glasses, union/polygon/rectangle obscurations)
+    is bit-identical to the oracle, flags included."""
+    import zemax_cases
+    from oracle.host import HostRays, HostPlan
+    from batoid_amd.plan import PlanSpec, collect_steps
+    optic = zemax_cases.optic(name)
+    for thx, thy, w in [(0.0, 0.0, 500e-9), (0.004, -0.006, 800e-9)]:
+        rv = bt.RayVector.asPolar(optic=optic, wavelength=w, theta_x=thx, theta_y=thy,
+                                  nrad=60, naz=360)
+        ref = HostRays.from_rv(rv)
+        HostPlan(PlanSpec(collect_steps(optic), rv.coordSys)).trace(ref)
+        optic.trace(rv)
+        for k in ("x", "y", "z", "vx", "vy", "vz", "t"):
+            a, b = getattr(rv, k), ref[k]
+            same = (a.view(np.uint64) == b.view(np.uint64)) | (np.isnan(a) & np.isnan(b))
+            assert same.all(), f"{name} {k}: {int((~same).sum())} rays differ"
+        for k in ("vignetted", "failed"):
+            np.testing.assert_array_equal(getattr(rv, k), ref[k].astype(bool))
+        assert 0 < int((~rv.vignetted).sum()) < len(rv)

@@ -183,3 +183,33 @@ def test_oracle_polar_centre_ray():
     t = -h["z"][-1] / h["vz"][-1]
     assert abs(h["x"][-1] + h["vx"][-1] * t) < 1e-14
     assert abs(h["y"][-1] + h["vy"][-1] * t) < 1e-14
+
+
+def test_hypot_restatement_is_libm():
+    """The device's ref_hypot algorithm (restated in oracle/libm.py) equals
+    x86-64 libm's hypot bit for bit -- on pupil-edge points, across the
+    rescaling ranges and at the special values -- while the correctly
+    naive sqrt(x*x + y*y) does not (so the check has teeth)."""
+    import ctypes
+    import math
+    from oracle import libm
+    c = ctypes.CDLL("libm.so.6")
+    c.hypot.restype = ctypes.c_double
+    c.hypot.argtypes = [ctypes.c_double, ctypes.c_double]
+    rng = np.random.default_rng(11)
+    th = rng.uniform(0, 2 * np.pi, 20000)
+    pts = [(r * math.cos(t), r * math.sin(t)) for r in (4.1, 4.18, 0.41, 1.0)
+           for t in th.tolist()]
+    e = rng.uniform(-330, 330, (20000, 2))
+    pts += list(zip((rng.normal(size=20000) * 10.0 ** np.clip(e[:, 0], -323, 307)).tolist(),
+                    (rng.normal(size=20000) * 10.0 ** np.clip(e[:, 1], -323, 307)).tolist()))
+    pts += [(0.0, 0.0), (-3.0, 4.0), (math.inf, math.nan), (math.nan, 2.0), (1e308, 1e308),
+            (5e-324, 5e-324), (0.0, -5e-324)]
+    cr_differs = 0
+    for x, y in pts:
+        a, b = c.hypot(x, y), libm.hypot(x, y)
+        assert a == b or (math.isnan(a) and math.isnan(b)), (x, y, a, b)
+        if math.isfinite(x) and math.isfinite(y) and 1e-150 < abs(x) < 1e150 \
+                and 1e-150 < abs(y) < 1e150:
+            cr_differs += math.sqrt(x * x + y * y) != a
+    assert cr_differs > 50
