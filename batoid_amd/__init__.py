@@ -21,5 +21,5 @@ from .lattice import Lattice
 from . import parse
 from . import utils
 from . import global_vars
-from .analysis import huygensPSF, wavefront, dkdu
+from .analysis import huygensPSF, wavefront, dkdu, zernike, zernikeGQ
 from ._lib import BatoidAMDError

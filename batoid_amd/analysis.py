@@ -1,5 +1,6 @@
-"""Huygens PSF, wavefront (OPD) and plate-scale Jacobian
-(batoid/analysis.py:10-56, 181-388) on the GPU.
+"""Huygens PSF, wavefront (OPD), its Zernike decomposition and the
+plate-scale Jacobian (batoid/analysis.py:10-56, 181-388, 479-641) on the
+GPU.
 
 ``huygensPSF`` keeps the reference's signature and lattice conventions.  The
 reference sums exp(i phi) over rays for every pixel in a Python loop
@@ -118,10 +119,10 @@ def huygensPSF(optic, theta_x, theta_y, wavelength, projection='postel', nx=None
     return Lattice(np.abs(amp) ** 2, primitiveX)
 
 
-def wavefront(optic, theta_x, theta_y, wavelength, projection='postel', nx=32,
-              sphereRadius=None, reference='chief'):
-    """Wavefront in waves against a reference sphere centred on the image
-    point (analysis.py:308-388)."""
+def _sphere_rays(optic, theta_x, theta_y, wavelength, projection, nx, sphereRadius,
+                 reference):
+    """The wavefront grid traced through the optic and onto the reference
+    sphere (analysis.py:342-373); returns the rays (device) and t0."""
     from .surface import Sphere
     dirCos = fieldToDirCos(theta_x, theta_y, projection=projection)
     rays = RayVector.asGrid(optic=optic, wavelength=wavelength, nx=nx, dirCos=dirCos)
@@ -132,13 +133,145 @@ def wavefront(optic, theta_x, theta_y, wavelength, projection='postel', nx=32,
     target = rays.coordSys.shiftLocal(point + np.array([0, 0, sphereRadius]))
     rays.toCoordSys(target)
     Sphere(-sphereRadius).intersect(rays)
-    t = rays.t
-    vig = rays.vignetted
     if reference == 'mean':
-        t0 = np.mean(t[np.where(1 - vig)[0]])
+        t = rays.t
+        t0 = np.mean(t[np.where(1 - rays.vignetted)[0]])
     else:
-        t0 = t[_chief_index(nx)]
-    arr = np.ma.masked_array((t0 - t) / wavelength, mask=vig).reshape(nx, nx)
+        t0 = float(rays.torch("t")[_chief_index(nx)].item())
+    return rays, t0
+
+
+def wavefront(optic, theta_x, theta_y, wavelength, projection='postel', nx=32,
+              sphereRadius=None, reference='chief'):
+    """Wavefront in waves against a reference sphere centred on the image
+    point (analysis.py:308-388)."""
+    rays, t0 = _sphere_rays(optic, theta_x, theta_y, wavelength, projection, nx,
+                            sphereRadius, reference)
+    arr = np.ma.masked_array((t0 - rays.t) / wavelength, mask=rays.vignetted).reshape(nx, nx)
     d = nx - 2 if (nx % 2) == 0 else nx - 1
     primitiveU = np.vstack([[optic.pupilSize / d, 0], [0, optic.pupilSize / d]])
     return Lattice(arr, primitiveU)
+
+
+def zernike_table(jmax, eps=0.0):
+    """Monomial coefficients of the Noll Zernikes Z_1 .. Z_jmax (unit outer
+    radius, obscuration ``eps``) as a [jmax][M] table, column
+    m = d(d+1)/2 + q for x^(d-q) y^q; returns (table, radial order)."""
+    from fractions import Fraction
+    from .zernike_poly import noll_to_nm, zernike_xy_exact
+    nmax = max(noll_to_nm(j)[0] for j in range(1, jmax + 1))
+    C = np.zeros((jmax, (nmax + 1) * (nmax + 2) // 2))
+    e = Fraction(float(eps)) if eps else Fraction(0)
+    for j in range(1, jmax + 1):
+        for (i, k), v in zernike_xy_exact(j, e).items():
+            d = i + k
+            C[j - 1, d * (d + 1) // 2 + k] = v
+    return C, nmax
+
+
+def zernike_gram(x, y, t, t0, wavelength, R_outer, R_inner=0.0, jmax=22, flux=None,
+                 mask=None):
+    """Gram matrix of [Z_1 .. Z_jmax, w] over the rays not masked, with
+    w = (t0 - t) / wavelength (times ``flux`` if given) and the Zernikes
+    evaluated at (x, y) / R_outer -- one bt_zernike_gram pass over device
+    columns.  Returns the (jmax+1, jmax+1) symmetric matrix (host)."""
+    if not 1 <= jmax <= 78:
+        raise ValueError("jmax must be in [1, 78]")
+    C, nmax = zernike_table(jmax, R_inner / R_outer if R_inner else 0.0)
+    lib = _lib.load()
+    dev = _device.device()
+    coef = _device.f64(C)
+    ws = torch.empty(int(lib.bt_zernike_workspace_bytes(jmax)), dtype=torch.uint8, device=dev)
+    nent = (jmax + 1) * (jmax + 2) // 2
+    out = torch.empty(nent, dtype=torch.float64, device=dev)
+    n = t.numel()
+    _lib.call("bt_zernike_gram", _device.ptr(x), _device.ptr(y), _device.ptr(t),
+              _device.ptr(flux), _device.ptr(mask), n, float(t0), float(wavelength),
+              float(R_outer), _device.ptr(coef), jmax, nmax, _device.ptr(out),
+              _device.ptr(ws), ws.numel(), _device.stream())
+    packed = _device.to_numpy(out)
+    G = np.zeros((jmax + 1, jmax + 1))
+    G[np.triu_indices(jmax + 1)] = packed
+    return G + np.triu(G, 1).T
+
+
+def zernike(optic, theta_x, theta_y, wavelength, projection='postel', nx=32,
+            sphereRadius=None, reference='chief', jmax=22, eps=0.0,
+            include_vignetted=False):
+    """Least-squares Zernike decomposition of the wavefront, in waves
+    (analysis.py:479-573).  The normal equations are accumulated on the GPU
+    (bt_zernike_gram) and the jmax x jmax system is solved on the host;
+    index 0 of the result is 0 (Noll indices start at 1)."""
+    dirCos = fieldToDirCos(theta_x, theta_y, projection=projection)
+    rays = RayVector.asGrid(optic=optic, wavelength=wavelength, nx=nx, dirCos=dirCos)
+    epRays = rays.toCoordSys(optic.stopSurface.coordSys)
+    optic.stopSurface.surface.intersect(epRays)
+    srays, t0 = _sphere_rays(optic, theta_x, theta_y, wavelength, projection, nx,
+                             sphereRadius, reference)
+    if include_vignetted:
+        d = nx - 2 if (nx % 2) == 0 else nx - 1
+        dx = optic.pupilSize / d
+        k = np.arange(nx) - nx // 2
+        cx, cy = np.meshgrid(k * dx, k * dx)
+        mask = _device.u8((np.hypot(cx, cy) > optic.pupilSize / 2).ravel())
+    else:
+        mask = srays.torch("vignetted")
+    G = zernike_gram(epRays.torch("x"), epRays.torch("y"), srays.torch("t"), t0, wavelength,
+                     optic.pupilSize / 2, optic.pupilSize / 2 * eps, jmax, mask=mask)
+    A, b = G[:jmax, :jmax], G[:jmax, jmax]
+    nrows = int((mask == 0).sum().item())
+    s = np.sqrt(np.clip(np.linalg.eigvalsh(A), 0.0, None))
+    rank = int((s > s.max() * max(nrows, jmax + 1) * np.finfo(float).eps).sum())
+    if rank < jmax:
+        raise RuntimeError(
+            f"Rank {rank} matrix found when fitting {jmax} Zernike coefficients.  ")
+    coefs = np.zeros(jmax + 1)
+    coefs[1:] = np.linalg.solve(A, b)
+    return coefs
+
+
+def zernikeGQ(optic, theta_x, theta_y, wavelength, projection='postel', rings=6,
+              spokes=None, sphereRadius=None, reference='chief', jmax=22, eps=0.0):
+    """Zernike decomposition by Gaussian quadrature over the (annular) pupil,
+    in waves (analysis.py:576-641): flux-weighted projections of the
+    wavefront onto each Zernike, one bt_zernike_gram pass."""
+    from .surface import Sphere
+    dirCos = fieldToDirCos(theta_x, theta_y, projection=projection)
+    inner = eps * optic.pupilSize / 2
+    rays = RayVector.asSpokes(optic=optic, wavelength=wavelength, inner=inner, dirCos=dirCos,
+                              rings=rings, spokes=spokes, spacing='GQ')
+    epRays = rays.copy().toCoordSys(optic.stopSurface.coordSys)
+    optic.stopSurface.surface.intersect(epRays)
+    if sphereRadius is None:
+        sphereRadius = optic.sphereRadius
+    optic.trace(rays)
+    if bool(rays.torch("failed").any()):
+        raise ValueError("Cannot compute zernike with Gaussian Quadrature with failed rays.")
+    if reference == 'mean':
+        w = np.where(1 - rays.vignetted)[0]
+        point = np.mean(rays.r[w], axis=0)
+    elif reference == 'chief':
+        chiefRay = RayVector.fromStop(0.0, 0.0, backDist=optic.backDist, wavelength=wavelength,
+                                      dirCos=dirCos, medium=optic.inMedium,
+                                      stopSurface=optic.stopSurface)
+        optic.trace(chiefRay)
+        point = chiefRay.r[0]
+    else:
+        raise ValueError(f"unknown reference {reference!r}")
+    target = rays.coordSys.shiftLocal(point + np.array([0, 0, sphereRadius]))
+    rays.toCoordSys(target)
+    sphere = Sphere(-sphereRadius)
+    sphere.intersect(rays)
+    if reference == 'mean':
+        w = np.where(1 - rays.vignetted)[0]
+        t0 = np.mean(rays.t[w])
+    else:
+        chiefRay.toCoordSys(target)
+        sphere.intersect(chiefRay)
+        t0 = float(chiefRay.t[0])
+    G = zernike_gram(epRays.torch("x"), epRays.torch("y"), rays.torch("t"), t0, wavelength,
+                     optic.pupilSize / 2, inner, jmax, flux=rays.torch("flux"))
+    area = np.pi * (1. - eps**2)
+    out = np.zeros(jmax + 1)
+    out[1:] = G[:jmax, jmax] / area
+    return out

@@ -15,7 +15,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libbatoid_amd.so")
-SOURCES = ["bt_trace.hip", "bt_field.hip", "bt_huygens.hip", "bt_mlp.hip", "bt_raygen.hip"]
+SOURCES = ["bt_trace.hip", "bt_field.hip", "bt_huygens.hip", "bt_mlp.hip", "bt_raygen.hip",
+           "bt_zernike.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"

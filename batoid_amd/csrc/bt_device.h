@@ -770,6 +770,26 @@ BT_DEV double ref_hypot(double x, double y)
     return hypot_kernel(ax, ay);
 }
 
+// The comparison hypot(x, y) < r decided from s = x*x + y*y without the
+// square root: -1 when hypot(x, y) < r for certain, +1 when > r, 0 when s
+// lies within 1e-13 of r*r (or outside the range where that bound holds:
+// NaN, inf, r outside {0} and [1e-140, 1e140], s >= 1e280), where only
+// libm's value decides.  s carries at most ~2^-51 relative error and libm's hypot
+// is within 1 ulp of the true value, far inside the 1e-13 margin, so a
+// decided comparison agrees with libm's.
+BT_DEV int cmp_hypot(double x, double y, double r)
+{
+    const double s = x*x + y*y;
+    if (r == 0.0) return (s > 0.0 && s < 1e280) ? 1 : 0;   // an annulus from 0
+    if (!(r > 1e-140 && r < 1e140)) return 0;
+    if (!(s < 1e280)) return 0;
+    const double r2 = r*r;
+    const double d = s - r2;
+    if (d > 1e-13*r2) return 1;
+    if (d < -1e-13*r2) return -1;
+    return 0;
+}
+
 // ---------------------------------------------------------------------------
 // Obscuration programs (src/obscuration.cpp); the stack is a bit mask.
 // ---------------------------------------------------------------------------
@@ -810,12 +830,21 @@ BT_DEV bool obsc_contains(const bt_obsc* prog, int len, double x, double y)
         const double* p = o->p;
         bool v;
         switch (o->op) {
-        case BT_OB_CIRCLE:                           // obscuration.cpp:32-34
-            v = ref_hypot(x-p[1], y-p[2]) < p[0];
+        case BT_OB_CIRCLE: {                         // obscuration.cpp:32-34
+            const double dx = x-p[1], dy = y-p[2];
+            const int c = cmp_hypot(dx, dy, p[0]);
+            v = c ? (c < 0) : (ref_hypot(dx, dy) < p[0]);
             break;
+        }
         case BT_OB_ANNULUS: {                        // obscuration.cpp:80-83
-            double h = ref_hypot(x-p[2], y-p[3]);
-            v = (p[0] <= h) && (h < p[1]);
+            const double dx = x-p[2], dy = y-p[3];
+            const int ci = cmp_hypot(dx, dy, p[0]), co = cmp_hypot(dx, dy, p[1]);
+            if (ci && co) {
+                v = (ci > 0) && (co < 0);
+            } else {
+                const double h = ref_hypot(dx, dy);
+                v = (p[0] <= h) && (h < p[1]);
+            }
             break;
         }
         case BT_OB_RECTANGLE: {                      // obscuration.cpp:117-121

@@ -6,6 +6,8 @@ plan and traces every ray through the whole optic in one fused GPU kernel
 (per-ray results identical to calling each Interface in turn)."""
 from functools import cached_property
 
+import numpy as np
+
 from . import _lib, _device
 from .constants import globalCoordSys, vacuum
 from .obscuration import ObscNegation, ObscCircle, ObscAnnulus
@@ -42,6 +44,55 @@ class Optic:
         from .parse import load_yaml, parse_optic
         return parse_optic(load_yaml(filename)['opticalSystem'])
 
+    # ---- rigid-body perturbations (optic.py:571-697, 1362-1566) ----------
+    def _moved(self, frame, stop, sub):
+        """Copy with coordSys -> frame(coordSys), stopSurface ->
+        stop(stopSurface) and, for compound optics, each item -> sub(item)."""
+        kw = _init_kwargs(self, (self._lead,))
+        kw['coordSys'] = frame(self.coordSys)
+        if kw.get('stopSurface') is not None:
+            kw['stopSurface'] = stop(kw['stopSurface'])
+        lead = getattr(self, self._lead)
+        if self._lead == 'items':
+            lead = [sub(item) for item in lead]
+        ret = self.__class__.__new__(self.__class__)
+        ret.__init__(lead, **kw)
+        return ret
+
+    def withGlobalShift(self, shift):
+        """Copy shifted by ``shift`` (global axes), stop surface and
+        sub-items included."""
+        shift = np.asarray(shift, dtype=float)
+        again = lambda o: o.withGlobalShift(shift)
+        return self._moved(lambda cs: cs.shiftGlobal(shift), again, again)
+
+    def withLocalShift(self, shift):
+        """Copy shifted by ``shift`` along this optic's own axes."""
+        return self.withGlobalShift(self.coordSys.rot @ np.asarray(shift, dtype=float))
+
+    def withGlobalRotation(self, rot, rotCenter=None, coordSys=None):
+        """Copy rotated by ``rot`` (global axes) about ``rotCenter`` given in
+        ``coordSys`` (default global; 'local' = this optic's frame)."""
+        center = [0, 0, 0] if rotCenter is None else rotCenter
+        if coordSys is None:
+            cs = globalCoordSys
+        elif isinstance(coordSys, str):
+            if coordSys != 'local':
+                raise ValueError(f"unknown coordSys {coordSys!r}")
+            cs = self.coordSys
+        else:
+            cs = coordSys
+        again = lambda o: o.withGlobalRotation(rot, center, cs)
+        return self._moved(lambda c: c.rotateGlobal(rot, center, cs), again, again)
+
+    def withLocalRotation(self, rot, rotCenter=None, coordSys=None):
+        """Copy rotated by ``rot`` about this optic's own axes, about
+        ``rotCenter`` given in ``coordSys`` (default this optic's frame)."""
+        center = [0, 0, 0] if rotCenter is None else rotCenter
+        cs = self.coordSys if coordSys is None else coordSys
+        again = lambda o: o.withLocalRotation(rot, center, cs)
+        return self._moved(lambda c: c.rotateLocal(rot, center, cs), again, again)
+
     def _repr_helper(self):
         out = ""
         if self.name is not None:
@@ -57,6 +108,7 @@ class Optic:
 class Interface(Optic):
     """A single surface with an optional obscuration; rays pass through
     (optic.py:98-140, 541-543)."""
+    _lead = 'surface'
 
     def __init__(self, surface, obscuration=None, **kwargs):
         Optic.__init__(self, **kwargs)
@@ -198,6 +250,7 @@ class OPDScreen(Interface):
 
 class CompoundOptic(Optic):
     """A sequence of optics traced in order (optic.py:933-1083)."""
+    _lead = 'items'
 
     def __init__(self, items, **kwargs):
         Optic.__init__(self, **kwargs)
@@ -252,14 +305,24 @@ class CompoundOptic(Optic):
         return tuple(flags)
 
     def _plan(self, coordSys_in, reverse, start=None):
+        """The device plan of the optic as it is now.  Items' surfaces,
+        obscurations, media and frames may be reassigned between traces
+        (``optic['M1'].obscuration = ...``), so the cache key names every
+        step's objects; the entry keeps those objects alive, so their ids
+        cannot be reused while the entry exists."""
         from .plan import PlanSpec, DevicePlan, collect_steps, start_key
-        key = (coordSys_in, reverse, self._skips(), start_key(start))
+        steps = collect_steps(self, reverse)
+        ident = tuple((id(i), id(i.surface), id(i.obscuration), id(m2), i.coordSys, action)
+                      for i, action, m2 in steps)
+        key = (coordSys_in, reverse, ident, start_key(start))
         cache = self.__dict__.setdefault('_plan_cache', {})
-        plan = cache.get(key)
-        if plan is None:
-            plan = DevicePlan(PlanSpec(collect_steps(self, reverse), coordSys_in, start))
-            cache[key] = plan
-        return plan
+        entry = cache.get(key)
+        if entry is None:
+            if len(cache) >= 16:                     # bounded: drop the oldest
+                cache.pop(next(iter(cache)))
+            entry = (DevicePlan(PlanSpec(steps, coordSys_in, start)), steps)
+            cache[key] = entry
+        return entry[0]
 
     def trace(self, rv, reverse=False, path=None, start=None):
         """Trace rays through every unskipped item, in place
@@ -351,6 +414,76 @@ class CompoundOptic(Optic):
 
     def withPerturbedSurface(self, name, perturbation):
         return self.withSurface(name, self[name].surface + perturbation)
+
+    def withLocalRotation(self, rot, rotCenter=None, coordSys=None):
+        """Copy rotated about this optic's own axes: the rotation
+        R rot R^T in global axes, applied to every sub-item."""
+        center = [0, 0, 0] if rotCenter is None else rotCenter
+        cs = self.coordSys if coordSys is None else coordSys
+        R = self.coordSys.rot
+        return self.withGlobalRotation(R @ rot @ R.T, center, cs)
+
+    def _full_name(self, name):
+        name = self._names.get(name, name)
+        if name not in self.itemDict:
+            raise ValueError("Optic {} not found".format(name))
+        return name
+
+    def _with_item(self, name, change):
+        """Copy with item ``name`` replaced by change(item) (the whole optic
+        when ``name`` is this optic)."""
+        name = self._full_name(name)
+        if name == self.name:
+            return change(self)
+        item = self.itemDict[name]
+        return self._replaced(item, change(item))
+
+    def withGloballyShiftedOptic(self, name, shift):
+        """Copy with item ``name`` shifted along global axes
+        (optic.py:1410-1457)."""
+        return self._with_item(name, lambda o: o.withGlobalShift(shift))
+
+    def withLocallyShiftedOptic(self, name, shift):
+        """Copy with item ``name`` shifted along its own axes
+        (optic.py:1459-1481)."""
+        item = self.itemDict[self._full_name(name)]
+        return self.withGloballyShiftedOptic(name, item.coordSys.rot @ np.asarray(shift, float))
+
+    def withGloballyRotatedOptic(self, name, rot, rotCenter=None, coordSys=None):
+        """Copy with item ``name`` rotated about global axes; ``rotCenter``
+        in ``coordSys`` (default global) (optic.py:1630-1693)."""
+        return self._with_item(
+            name, lambda o: o.withGlobalRotation(rot, rotCenter,
+                                                 globalCoordSys if coordSys is None
+                                                 else coordSys))
+
+    def withLocallyRotatedOptic(self, name, rot, rotCenter=None, coordSys=None):
+        """Copy with item ``name`` rotated about its own axes; ``rotCenter``
+        in ``coordSys`` (default the item's frame) (optic.py:1567-1628)."""
+        item = self.itemDict[self._full_name(name)]
+        cs = item.coordSys if coordSys is None else coordSys
+        return self._with_item(name, lambda o: o.withLocalRotation(rot, rotCenter, cs))
+
+    def withInsertedOptic(self, before, item):
+        """Copy with ``item`` inserted before the direct sub-item ``before``
+        (optic.py:1760-1790)."""
+        target = self._full_name(before)
+        items = []
+        for it in self.items:
+            if self._names.get(it.name) == target:
+                items.append(item)
+            items.append(it)
+        ret = self.__class__.__new__(self.__class__)
+        ret.__init__(items, **_init_kwargs(self, ('items',)))
+        return ret
+
+    def withRemovedOptic(self, item):
+        """Copy without the direct sub-item ``item`` (optic.py:1792-1818)."""
+        target = self._full_name(item)
+        items = [it for it in self.items if self._names.get(it.name) != target]
+        ret = self.__class__.__new__(self.__class__)
+        ret.__init__(items, **_init_kwargs(self, ('items',)))
+        return ret
 
     def __eq__(self, other):
         if not self.__class__ == other.__class__:

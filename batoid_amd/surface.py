@@ -10,7 +10,7 @@ import numpy as np
 
 from . import _lib
 from ._cache import DeviceCached
-from .zernike import zernike_coef_arrays
+from .zernike_poly import zernike_coef_arrays
 
 
 def _f(x):

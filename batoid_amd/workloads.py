@@ -150,7 +150,7 @@ def _asphere_sag(asph, x, y):
 def _fit_zernike(fun, R_outer, R_inner, jmax, npts=20000, seed=1):
     """Least-squares Zernike coefficients (Noll 1..jmax, circular basis over
     R_outer) of fun(x, y) sampled uniformly over the annulus."""
-    from .zernike import zernike_coef_arrays
+    from .zernike_poly import zernike_coef_arrays
     rng = np.random.default_rng(seed)
     r = np.sqrt(rng.uniform(R_inner ** 2, R_outer ** 2, npts))
     th = rng.uniform(0, 2 * np.pi, npts)

@@ -447,6 +447,26 @@ int bt_huygens_batch(bt_rays rays, const bt_psf_segment* segments, int nseg, int
                      double* amp_out, void* workspace, int64_t workspace_bytes,
                      void* stream);
 
+/* ---- Zernike decomposition of a wavefront --------------------------------
+ * The O(N) part of batoid.zernike / zernikeGQ (batoid/analysis.py:479-641;
+ * the reference evaluates galsim's zernikeBasis and calls numpy's lstsq or
+ * dot on the host).  Over the rays with mask[i] == 0 (mask NULL: all rays):
+ *   w_i = (t0 - t[i]) / wavelength            (analysis.py:375, 639)
+ *   w_i *= flux[i]                             if flux != NULL (quadrature)
+ *   B_j(i) = sum_m coef[j][m] u^(d-q) v^q      u = x/r_outer, v = y/r_outer,
+ *                                              m = d(d+1)/2 + q, d <= nmax,
+ * for j = 0 .. jmax-1 (Noll j+1), and the Gram matrix of [B_0 .. B_jmax-1, w]
+ * is written packed by rows (upper triangle, (jmax+1)(jmax+2)/2 doubles):
+ * the normal matrix, the projections sum B_j w and sum w^2.  coef is a
+ * device [jmax][(nmax+1)(nmax+2)/2] table of monomial coefficients;
+ * jmax <= 78, nmax <= 11.  Summation order is fixed (run-to-run
+ * deterministic).  Workspace: bt_zernike_workspace_bytes(jmax) bytes.    */
+int64_t bt_zernike_workspace_bytes(int jmax);
+int bt_zernike_gram(const double* x, const double* y, const double* t, const double* flux,
+                    const uint8_t* mask, int64_t n, double t0, double wavelength,
+                    double r_outer, const double* coef, int jmax, int nmax, double* gram,
+                    void* workspace, int64_t workspace_bytes, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -116,7 +116,7 @@ def copies():
         shutil.copy(os.path.join(TD, fn), os.path.join(OUT, fn))
     os.makedirs(os.path.join(OUT, "optics"), exist_ok=True)
     for sub, fn in [("LSST", "LSST_g_500.yaml"), ("LSST", "LSST_r.yaml"),
-                    ("HSC", "HSC_old.yaml"), ("HSC", "HSC_no_obsc.yaml"),
+                    ("HSC", "HSC.yaml"), ("HSC", "HSC_old.yaml"), ("HSC", "HSC_no_obsc.yaml"),
                     ("DECam", "DECam.yaml"), ("LSST", "CBP.yaml")] + [
                         ("LSST", f"ComCam_{b}.yaml") for b in "ugrizy"]:
         shutil.copy(os.path.join(REF, "batoid", "data", sub, fn),

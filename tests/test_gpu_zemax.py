@@ -111,3 +111,31 @@ def test_fused_plan_bit_exact_on_golden_optics(name):
         for k in ("vignetted", "failed"):
             np.testing.assert_array_equal(getattr(rv, k), ref[k].astype(bool))
         assert 0 < int((~rv.vignetted).sum()) < len(rv)
+
+
+def test_plan_follows_reassigned_items():
+    """Reassigning an item's obscuration or surface between traces (as the
+    reference's own tests do) is honoured by the next fused trace."""
+    from oracle.host import HostRays, HostPlan
+    from batoid_amd.plan import PlanSpec, collect_steps
+    telescope = bt.Optic.fromYaml(os.path.join(GOLD, "optics", "LSST_r.yaml"))
+
+    def check():
+        rv = bt.RayVector.asPolar(optic=telescope, wavelength=625e-9, theta_x=0.0,
+                                  theta_y=0.0, nrad=40, naz=200, inner=0.0)
+        ref = HostRays.from_rv(rv)
+        HostPlan(PlanSpec(collect_steps(telescope), rv.coordSys)).trace(ref)
+        telescope.trace(rv)
+        for k in ("x", "y", "z", "t"):
+            np.testing.assert_array_equal(getattr(rv, k), ref[k])
+        np.testing.assert_array_equal(rv.vignetted, ref["vignetted"].astype(bool))
+        return int(rv.vignetted.sum())
+
+    n0 = check()
+    telescope.clearObscuration()
+    n1 = check()
+    telescope['LSST.M1'].obscuration = bt.ObscNegation(bt.ObscAnnulus(0.61 * 4.18, 4.18))
+    n2 = check()
+    telescope['LSST.M2'].surface = bt.Quadric(6.788, -0.2)
+    check()
+    assert n1 < n2 <= n0, (n0, n1, n2)

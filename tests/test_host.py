@@ -11,7 +11,7 @@ import oracle
 from oracle.host import host_records
 from batoid_amd import _lib
 from batoid_amd.plan import PlanSpec, collect_steps
-from batoid_amd.zernike import noll_to_nm, zernike_coef_arrays
+from batoid_amd.zernike_poly import noll_to_nm, zernike_coef_arrays
 from batoid_amd import workloads
 
 GOLD = os.path.join(os.path.dirname(__file__), "golden")
@@ -182,3 +182,65 @@ def test_field_projections_round_trip():
         np.testing.assert_allclose(a * a + b * b + c * c, 1, rtol=1e-14)
         np.testing.assert_allclose(bt.utils.dirCosToField(a, b, c, projection=proj), (u, v),
                                    rtol=1e-12)
+
+
+def _hsc():
+    return bt.Optic.fromYaml(os.path.join(os.path.dirname(__file__), "golden", "optics",
+                                          "HSC.yaml"))
+
+
+def test_shift_round_trips():
+    """tests/test_Optic.py:134-166: global and local shifts of the whole
+    optic and of every item (fully and partially qualified names) undo
+    exactly."""
+    rng = np.random.default_rng(5)
+    telescope = _hsc()
+    shift = rng.uniform(low=-1, high=1, size=3)
+    assert telescope.withGlobalShift(shift).withGlobalShift(-shift) == telescope
+    for item in telescope.itemDict:
+        shifted = telescope.withGloballyShiftedOptic(item, shift)
+        assert telescope == shifted.withGloballyShiftedOptic(item, -shift)
+    shifted = telescope.withGloballyShiftedOptic("G1", shift)
+    assert shifted.withGloballyShiftedOptic("G1", -shift) == telescope
+    assert telescope.withLocalShift(shift).withLocalShift(-shift) == telescope
+    assert (telescope.withGloballyShiftedOptic(telescope.name, shift)
+            == telescope.withGlobalShift(shift))
+    for item in telescope.itemDict:
+        shifted = telescope.withLocallyShiftedOptic(item, shift)
+        assert telescope == shifted.withLocallyShiftedOptic(item, -shift)
+    assert (telescope.withLocallyShiftedOptic(telescope.name, shift)
+            == telescope.withLocalShift(shift))
+    # a shifted item really moved, and nothing else did
+    moved = telescope.withGloballyShiftedOptic("PM", shift)
+    np.testing.assert_allclose(moved["PM"].coordSys.origin,
+                               telescope["PM"].coordSys.origin + shift)
+    assert moved["G1"] == telescope["G1"]
+
+
+def test_local_shift_with_rotation():
+    """tests/test_Optic.py:250-291."""
+    telescope = _hsc()
+    rotFull = telescope.withLocalRotation(bt.RotX(np.pi / 2))
+    g = rotFull.withGlobalShift([1, 3, 5])
+    loc = rotFull.withLocalShift([1, 5, -3])
+    assert np.allclose(g.coordSys.origin, loc.coordSys.origin)
+    assert np.allclose(g.coordSys.rot, loc.coordSys.rot)
+    rotCAM = telescope.withLocallyRotatedOptic("CAM", bt.RotY(np.pi / 2))
+    g = rotCAM.withGloballyShiftedOptic("CAM", [3, 5, 1])
+    loc = rotCAM.withLocallyShiftedOptic("CAM", [-1, 5, 3])
+    assert np.allclose(g["CAM"].coordSys.origin, loc["CAM"].coordSys.origin)
+    assert np.allclose(g["CAM"].coordSys.rot, loc["CAM"].coordSys.rot)
+
+
+def test_insert_and_remove_optic():
+    telescope = bt.Optic.fromYaml(os.path.join(os.path.dirname(__file__), "golden", "optics",
+                                               "LSST_r.yaml"))
+    names = [it.name for it in telescope.items]
+    extra = bt.Baffle(bt.Plane(), name="extraBaffle",
+                      coordSys=bt.CoordSys(origin=[0, 0, 3.0]))
+    ins = telescope.withInsertedOptic(names[1], extra)
+    assert [it.name for it in ins.items] == names[:1] + ["extraBaffle"] + names[1:]
+    rem = ins.withRemovedOptic("extraBaffle")
+    assert rem == telescope
+    with pytest.raises(ValueError):
+        telescope.withRemovedOptic("noSuchItem")
