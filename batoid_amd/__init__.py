@@ -18,6 +18,7 @@ from .trace import (intersect, applyForwardTransform, applyReverseTransform, obs
 from .optic import (Optic, CompoundOptic, Lens, Interface, RefractiveInterface, Mirror,
                     Detector, Baffle, OPDScreen)
 from .lattice import Lattice
+from .coating import Coating, SimpleCoating
 from . import parse
 from . import utils
 from . import global_vars

@@ -95,7 +95,12 @@ class BtRaygen(ctypes.Structure):
     ]
 
 
-STRUCTS = (BtRays, BtSurface, BtMedium, BtObsc, BtStep, BtPlan, BtPsfSegment, BtRaygen)
+class BtCoating(ctypes.Structure):
+    _fields_ = [("reflectivity", ctypes.c_double), ("transmissivity", ctypes.c_double)]
+
+
+STRUCTS = (BtRays, BtSurface, BtMedium, BtObsc, BtStep, BtPlan, BtPsfSegment, BtRaygen,
+           BtCoating)
 
 # enum values (include/batoid_amd.h)
 PLANE, SPHERE, PARABOLOID, QUADRIC, ASPHERE, TILTED, BICUBIC, POLYNOMIAL, SUM = range(9)
@@ -130,9 +135,13 @@ SIGNATURES = {
     "bt_apply_reverse_transform_arrays": (_I, [_V, _V, _V, _V, _V, _L, _V]),
     "bt_finish_parallel": (_I, [_V, _V, _V, _V, _V, _V, _L, _V]),
     "bt_generate_rays": (_I, [_V, BtRays, _V]),
-    "bt_intersect": (_I, [_V, _V, _V, BtRays, _I, _V, _I, _V]),
-    "bt_reflect": (_I, [_V, _V, _V, BtRays, _I, _V, _I, _V]),
-    "bt_refract": (_I, [_V, _V, _V, _V, BtRays, _I, _V, _I, _V]),
+    "bt_intersect": (_I, [_V, _V, _V, BtRays, _I, _V, _V, _I, _V]),
+    "bt_reflect": (_I, [_V, _V, _V, BtRays, _I, _V, _V, _I, _V]),
+    "bt_refract": (_I, [_V, _V, _V, _V, BtRays, _I, _V, _V, _I, _V]),
+    "bt_rsplit": (_I, [_V, _V, _V, _V, _V, BtRays, BtRays, _I, _V]),
+    "bt_select_workspace_bytes": (_L, [_L]),
+    "bt_select_count": (_I, [BtRays, _D, _V, _V, _L, _V]),
+    "bt_select_scatter": (_I, [BtRays, _D, BtRays, _V, _L, _V]),
     "bt_obscure": (_I, [_V, _I, BtRays, _V]),
     "bt_trace_plan": (_I, [BtPlan, BtRays, BtRays, _I, _V]),
     "bt_surface_sag": (_I, [_V, _V, _V, _V, _L, _V]),

@@ -16,7 +16,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libbatoid_amd.so")
 SOURCES = ["bt_trace.hip", "bt_field.hip", "bt_huygens.hip", "bt_mlp.hip", "bt_raygen.hip",
-           "bt_zernike.hip"]
+           "bt_zernike.hip", "bt_split.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"

@@ -1,0 +1,278 @@
+// bt_split.hip -- ghost splitting: rSplit and the ray selection of traceSplit.
+//
+// bt_rsplit splits every ray that reaches a surface into its refracted and
+// reflected parts in one pass (src/batoid.cpp:509-674): lane i intersects
+// ray i with the reference's Newton iteration, then writes the reflected
+// ray to the split buffer and the refracted ray back in place, with the
+// coating's flux fractions.  The arithmetic is the reference's, operation by
+// operation.
+//
+// traceSplit (batoid/optic.py:1244-1282) keeps, after every Interface, only
+// the rays that are unvignetted and above a flux threshold.  Here that
+// filter is an order-preserving stream compaction: per-block counts from
+// wave ballots, one exclusive scan over the blocks, then a scatter that
+// recomputes each ray's rank from the same ballots.
+#include <hip/hip_runtime.h>
+#include <string>
+
+#include "bt_device.h"
+#include "bt_common.h"
+
+namespace bt {
+
+__global__ __launch_bounds__(kBlock) void k_rsplit(
+    Xform xf, const bt_surface* __restrict__ surf, const bt_medium* __restrict__ m2,
+    double reflect, double transmit, bt_rays rv, bt_rays sp, int niter)
+{                                                    // batoid.cpp:582-672
+    const int64_t i = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (i >= rv.n) return;
+    const double* dr = xf.dr;
+    const double* drot = xf.drot;
+    const double x0 = rv.x[i], y0 = rv.y[i], z0 = rv.z[i];
+    const double vx0 = rv.vx[i], vy0 = rv.vy[i], vz0 = rv.vz[i];
+    const double t0 = rv.t[i], w = rv.wavelength[i], f = rv.flux[i];
+    const bool vig0 = rv.vignetted[i] != 0, fail0 = rv.failed[i] != 0;
+    // the split ray starts as a copy of the input (trace.py:125)
+    double sx = x0, sy = y0, sz = z0, svx = vx0, svy = vy0, svz = vz0, st = t0, sf = f;
+    bool svig = vig0, sfail = fail0;
+    if (!fail0) {
+        double dx = x0-dr[0];
+        double dy = y0-dr[1];
+        double dz = z0-dr[2];
+        double x = dx*drot[0] + dy*drot[3] + dz*drot[6];
+        double y = dx*drot[1] + dy*drot[4] + dz*drot[7];
+        double z = dx*drot[2] + dy*drot[5] + dz*drot[8];
+        double vx = vx0*drot[0] + vy0*drot[3] + vz0*drot[6];
+        double vy = vx0*drot[1] + vy0*drot[4] + vz0*drot[7];
+        double vz = vx0*drot[2] + vy0*drot[5] + vz0*drot[8];
+        double t = t0;
+        double dt = 0.0;
+        NormalMemo memo;
+        memo.ok = false;
+        const bool hit = surf_tti(surf, x, y, z, vx, vy, vz, dt, niter, &memo);
+        if (hit) {
+            x += vx * dt;
+            y += vy * dt;
+            z += vz * dt;
+            t += dt;
+            double n1 = vx*vx;
+            n1 += vy*vy;
+            n1 += vz*vz;
+            n1 = 1/sqrt(n1);
+            double nvx = vx*n1;
+            double nvy = vy*n1;
+            double nvz = vz*n1;
+            double nx, ny, nz;
+            if (memo.ok && same_bits(memo.dt, dt)) {  // Newton's last normal, same point
+                nx = memo.nx; ny = memo.ny; nz = memo.nz;
+            } else {
+                surf_normal(surf, x, y, nx, ny, nz);
+            }
+            double alpha = nvx*nx;
+            alpha += nvy*ny;
+            alpha += nvz*nz;
+            if (alpha > 0) {
+                nx *= -1;
+                ny *= -1;
+                nz *= -1;
+                alpha *= -1;
+            }
+            // reflection
+            sx = x; sy = y; sz = z;
+            svx = vx - 2*alpha*nx/n1;
+            svy = vy - 2*alpha*ny/n1;
+            svz = vz - 2*alpha*nz/n1;
+            st = t;
+            sf = f*reflect;
+            // refraction
+            const double n2 = medium_n(m2, w);
+            const double eta = n1/n2;
+            const double sinsqr = eta*eta*(1-alpha*alpha);
+            rv.x[i] = x;
+            rv.y[i] = y;
+            rv.z[i] = z;
+            rv.t[i] = t;
+            if (sinsqr > 1.0) {                      // total internal reflection
+                const double nan = __builtin_nan("");
+                rv.vx[i] = nan; rv.vy[i] = nan; rv.vz[i] = nan;
+                rv.failed[i] = 1;
+                rv.vignetted[i] = 1;
+            } else {
+                const double nfactor = eta*alpha + sqrt(1-sinsqr);
+                double ux = eta*nvx - nfactor*nx;
+                double uy = eta*nvy - nfactor*ny;
+                double uz = eta*nvz - nfactor*nz;
+                rv.vx[i] = ux / n2;
+                rv.vy[i] = uy / n2;
+                rv.vz[i] = uz / n2;
+            }
+            rv.flux[i] = f*transmit;
+        } else {
+            rv.failed[i] = 1;
+            rv.vignetted[i] = 1;
+            svig = true;
+            sfail = true;
+        }
+    }
+    sp.x[i] = sx; sp.y[i] = sy; sp.z[i] = sz;
+    sp.vx[i] = svx; sp.vy[i] = svy; sp.vz[i] = svz;
+    sp.t[i] = st;
+    sp.wavelength[i] = w;
+    sp.flux[i] = sf;
+    sp.vignetted[i] = svig;
+    sp.failed[i] = sfail;
+}
+
+// ---------------------------------------------------------------------------
+// Order-preserving selection: block b covers rays [256 b, 256 b + 256).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool selected(const bt_rays& in, int64_t i, double min_flux)
+{                                                    // optic.py:1262
+    return i < in.n && in.vignetted[i] == 0 && in.flux[i] >= min_flux;
+}
+
+// Rank of this lane among the block's selected rays, and the block total.
+__device__ __forceinline__ int block_rank(bool keep, int& total)
+{
+    __shared__ int wave_count[kBlock / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint64_t m = __ballot(keep);
+    if (lane == 0) wave_count[wave] = __popcll(m);
+    __syncthreads();
+    int base = 0;
+    total = 0;
+    for (int k = 0; k < kBlock / 64; k++) {
+        if (k < wave) base += wave_count[k];
+        total += wave_count[k];
+    }
+    return base + __popcll(m & ((1ull << lane) - 1ull));
+}
+
+__global__ __launch_bounds__(kBlock) void k_select_count(bt_rays in, double min_flux,
+                                                         int64_t* __restrict__ counts)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    int total;
+    block_rank(selected(in, i, min_flux), total);
+    if (threadIdx.x == 0) counts[blockIdx.x] = total;
+}
+
+// Exclusive scan of the block counts (one workgroup, chunks of 256 in order).
+__global__ __launch_bounds__(kBlock) void k_select_scan(const int64_t* __restrict__ counts,
+                                                        int64_t nblocks,
+                                                        int64_t* __restrict__ offsets,
+                                                        int64_t* __restrict__ total)
+{
+    __shared__ int64_t buf[kBlock];
+    int64_t carry = 0;
+    for (int64_t base = 0; base < nblocks; base += kBlock) {
+        const int64_t b = base + threadIdx.x;
+        const int64_t c = b < nblocks ? counts[b] : 0;
+        buf[threadIdx.x] = c;
+        __syncthreads();
+        for (int d = 1; d < kBlock; d <<= 1) {       // inclusive Hillis-Steele scan
+            const int64_t v = threadIdx.x >= d ? buf[threadIdx.x - d] : 0;
+            __syncthreads();
+            buf[threadIdx.x] += v;
+            __syncthreads();
+        }
+        if (b < nblocks) offsets[b] = carry + buf[threadIdx.x] - c;
+        const int64_t chunk = buf[kBlock - 1];
+        __syncthreads();
+        carry += chunk;
+    }
+    if (threadIdx.x == 0) *total = carry;
+}
+
+__global__ __launch_bounds__(kBlock) void k_select_scatter(bt_rays in, double min_flux,
+                                                           const int64_t* __restrict__ offsets,
+                                                           bt_rays out)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const bool keep = selected(in, i, min_flux);
+    int total;
+    const int rank = block_rank(keep, total);
+    if (!keep) return;
+    const int64_t o = offsets[blockIdx.x] + rank;
+    if (o >= out.n) return;                          // out smaller than the count
+    out.x[o] = in.x[i]; out.y[o] = in.y[i]; out.z[o] = in.z[i];
+    out.vx[o] = in.vx[i]; out.vy[o] = in.vy[i]; out.vz[o] = in.vz[i];
+    out.t[o] = in.t[i];
+    out.wavelength[o] = in.wavelength[i];
+    out.flux[o] = in.flux[i];
+    out.vignetted[o] = in.vignetted[i];
+    out.failed[o] = in.failed[i];
+}
+
+int64_t select_blocks(int64_t n) { return (n + kBlock - 1) / kBlock; }
+
+}  // namespace bt
+
+using namespace bt;
+
+extern "C" {
+
+int bt_rsplit(const bt_surface* surface, const double* dr, const double* drot,
+              const bt_medium* m2, const bt_coating* coating, bt_rays rays,
+              bt_rays split, int niter, void* stream)
+{
+    if (!surface || !dr || !drot || !m2 || !coating)
+        return fail(BT_E_NULL, "bt_rsplit: null argument");
+    if (niter < 0) return fail(BT_E_SIZE, "bt_rsplit: negative niter");
+    if (rays.n < 0 || split.n < rays.n)
+        return fail(BT_E_SIZE, "bt_rsplit: split holds fewer rays");
+    if (rays.n == 0) return 0;
+    if (!rays_ok(rays) || !rays_ok(split))
+        return fail(BT_E_NULL, "bt_rsplit: incomplete ray columns");
+    hipLaunchKernelGGL(k_rsplit, dim3(grid_for(rays.n)), dim3(kBlock), 0, as_stream(stream),
+                       make_xform(dr, drot), surface, m2, coating->reflectivity,
+                       coating->transmissivity, rays, split, niter);
+    return check_launch("bt_rsplit");
+}
+
+int64_t bt_select_workspace_bytes(int64_t n)
+{
+    if (n < 0) return -1;
+    return 2 * (select_blocks(n) + 1) * (int64_t)sizeof(int64_t);
+}
+
+int bt_select_count(bt_rays in, double min_flux, int64_t* count, void* workspace,
+                    int64_t workspace_bytes, void* stream)
+{
+    if ((in.n > 0 && !rays_ok(in)) || in.n < 0 || !count || !workspace)
+        return fail(BT_E_NULL, "bt_select_count: null argument");
+    if (workspace_bytes < bt_select_workspace_bytes(in.n))
+        return fail(BT_E_SIZE, "bt_select_count: workspace too small");
+    const int64_t nb = select_blocks(in.n);
+    int64_t* counts = static_cast<int64_t*>(workspace);
+    int64_t* offsets = counts + nb + 1;
+    hipStream_t s = as_stream(stream);
+    if (nb > 0) {
+        if (nb > 0x7fffffff) return fail(BT_E_SIZE, "bt_select_count: too many rays");
+        hipLaunchKernelGGL(k_select_count, dim3((unsigned)nb), dim3(kBlock), 0, s, in,
+                           min_flux, counts);
+        const int rc = check_launch("bt_select_count");
+        if (rc) return rc;
+    }
+    hipLaunchKernelGGL(k_select_scan, dim3(1), dim3(kBlock), 0, s, (const int64_t*)counts, nb,
+                       offsets, count);
+    return check_launch("bt_select_count");
+}
+
+int bt_select_scatter(bt_rays in, double min_flux, bt_rays out, const void* workspace,
+                      int64_t workspace_bytes, void* stream)
+{
+    if ((in.n > 0 && !rays_ok(in)) || in.n < 0 || !workspace ||
+        (out.n > 0 && !rays_ok(out)))
+        return fail(BT_E_NULL, "bt_select_scatter: null argument");
+    if (workspace_bytes < bt_select_workspace_bytes(in.n))
+        return fail(BT_E_SIZE, "bt_select_scatter: workspace too small");
+    const int64_t nb = select_blocks(in.n);
+    if (nb == 0 || out.n == 0) return 0;
+    const int64_t* offsets = static_cast<const int64_t*>(workspace) + nb + 1;
+    hipLaunchKernelGGL(k_select_scatter, dim3((unsigned)nb), dim3(kBlock), 0,
+                       as_stream(stream), in, min_flux, offsets, out);
+    return check_launch("bt_select_scatter");
+}
+
+}  // extern "C"

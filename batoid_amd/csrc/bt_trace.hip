@@ -131,11 +131,13 @@ __global__ __launch_bounds__(kBlock) void k_finish_parallel(
     z[i] = xx*m[6] + yy*m[7] + zz*m[8] + dr[2];
 }
 
-// One Interface over all rays, in place.
+// One Interface over all rays, in place.  coat_on / coat: the coating's
+// flux factor for rays that reach the surface (batoid.cpp:251-263, 362-371,
+// 497-499); a SimpleCoating's factor does not depend on the incidence.
 __global__ __launch_bounds__(kBlock) void k_step(
     int action, Xform xf, const bt_surface* __restrict__ surf,
     const bt_medium* __restrict__ m2, const bt_obsc* __restrict__ obsc,
-    int obsc_len, bt_rays rv, int niter)
+    int obsc_len, bt_rays rv, int niter, int coat_on, double coat)
 {
     const int64_t i = ray_index();
     if (i >= rv.n) return;
@@ -156,6 +158,7 @@ __global__ __launch_bounds__(kBlock) void k_step(
         rv.x[i] = r.x; rv.y[i] = r.y; rv.z[i] = r.z;
         rv.vx[i] = r.vx; rv.vy[i] = r.vy; rv.vz[i] = r.vz;
         rv.t[i] = r.t;
+        if (coat_on) rv.flux[i] *= coat;
     }
     if (obsc_len > 0 && !r.vig)                     // batoid.cpp:169-171
         r.vig = obsc_contains(obsc, obsc_len, r.x, r.y);
@@ -345,8 +348,8 @@ bool valid_surface_kind(int k) { return k >= BT_PLANE && k <= BT_SUM; }
 
 int step_launch(int action, const bt_surface* surface, const double* dr,
                 const double* drot, const bt_medium* m2, bt_rays rays,
-                int niter, const bt_obsc* obsc, int obsc_len, void* stream,
-                const char* name)
+                int niter, const bt_coating* coating, const bt_obsc* obsc,
+                int obsc_len, void* stream, const char* name)
 {
     if (!surface || !dr || !drot) return fail(BT_E_NULL, std::string(name) + ": null argument");
     if (!rays_ok(rays)) return fail(BT_E_NULL, std::string(name) + ": incomplete ray columns");
@@ -355,8 +358,12 @@ int step_launch(int action, const bt_surface* surface, const double* dr,
         return fail(BT_E_SIZE, std::string(name) + ": bad obscuration program");
     if (niter < 0) return fail(BT_E_SIZE, std::string(name) + ": negative niter");
     if (rays.n == 0) return 0;
+    const double coat = !coating ? 1.0
+                        : (action == BT_ACT_REFLECT ? coating->reflectivity
+                                                    : coating->transmissivity);
     hipLaunchKernelGGL(k_step, dim3(grid_for(rays.n)), dim3(kBlock), 0, as_stream(stream),
-                       action, make_xform(dr, drot), surface, m2, obsc, obsc_len, rays, niter);
+                       action, make_xform(dr, drot), surface, m2, obsc, obsc_len, rays, niter,
+                       coating ? 1 : 0, coat);
     return check_launch(name);
 }
 
@@ -375,12 +382,13 @@ const char* bt_last_error(void) { return g_last_error.c_str(); }
 
 int bt_struct_sizes(int64_t* sizes, int n)
 {
-    const int64_t all[8] = {(int64_t)sizeof(bt_rays), (int64_t)sizeof(bt_surface),
+    const int64_t all[9] = {(int64_t)sizeof(bt_rays), (int64_t)sizeof(bt_surface),
                             (int64_t)sizeof(bt_medium), (int64_t)sizeof(bt_obsc),
                             (int64_t)sizeof(bt_step), (int64_t)sizeof(bt_plan),
-                            (int64_t)sizeof(bt_psf_segment), (int64_t)sizeof(bt_raygen)};
+                            (int64_t)sizeof(bt_psf_segment), (int64_t)sizeof(bt_raygen),
+                            (int64_t)sizeof(bt_coating)};
     if (!sizes) return fail(BT_E_NULL, "bt_struct_sizes: null argument");
-    for (int i = 0; i < n && i < 8; i++) sizes[i] = all[i];
+    for (int i = 0; i < n && i < 9; i++) sizes[i] = all[i];
     return 0;
 }
 
@@ -464,26 +472,29 @@ int bt_finish_parallel(const double* dr, const double* drot, const double* vv,
 
 int bt_intersect(const bt_surface* surface, const double* dr,
                  const double* drot, bt_rays rays, int niter,
-                 const bt_obsc* obsc, int obsc_len, void* stream)
+                 const bt_coating* coating, const bt_obsc* obsc, int obsc_len,
+                 void* stream)
 {
     return step_launch(BT_ACT_INTERSECT, surface, dr, drot, nullptr, rays, niter,
-                       obsc, obsc_len, stream, "bt_intersect");
+                       coating, obsc, obsc_len, stream, "bt_intersect");
 }
 
 int bt_reflect(const bt_surface* surface, const double* dr,
                const double* drot, bt_rays rays, int niter,
-               const bt_obsc* obsc, int obsc_len, void* stream)
+               const bt_coating* coating, const bt_obsc* obsc, int obsc_len,
+               void* stream)
 {
     return step_launch(BT_ACT_REFLECT, surface, dr, drot, nullptr, rays, niter,
-                       obsc, obsc_len, stream, "bt_reflect");
+                       coating, obsc, obsc_len, stream, "bt_reflect");
 }
 
 int bt_refract(const bt_surface* surface, const double* dr,
                const double* drot, const bt_medium* m2, bt_rays rays,
-               int niter, const bt_obsc* obsc, int obsc_len, void* stream)
+               int niter, const bt_coating* coating, const bt_obsc* obsc,
+               int obsc_len, void* stream)
 {
     return step_launch(BT_ACT_REFRACT, surface, dr, drot, m2, rays, niter,
-                       obsc, obsc_len, stream, "bt_refract");
+                       coating, obsc, obsc_len, stream, "bt_refract");
 }
 
 int bt_obscure(const bt_obsc* obsc, int obsc_len, bt_rays rays, void* stream)

@@ -32,7 +32,7 @@ constexpr int kZMaxJ = 78;                                 // basis functions
 constexpr int kZMaxEnt = (kZMaxJ + 1) * (kZMaxJ + 2) / 2;  // 3160 Gram entries
 constexpr int kZThreads = 256;
 constexpr int kZPer = (kZMaxEnt + kZThreads - 1) / kZThreads;   // 13 per thread
-constexpr int kZGrid = 256;
+constexpr int kZGrid = 512;   // two 81 KB-LDS workgroups per CU
 
 struct ZArgs {
     const double* x;
@@ -100,9 +100,18 @@ __global__ __launch_bounds__(kZThreads) void k_zernike_gram(ZArgs a)
         for (int e = tid; e < a.J * kZChunk; e += kZThreads) {
             const int j = e / kZChunk, p = e % kZChunk;
             const double* cj = a.coef + (int64_t)j * a.M;
-            double s = 0.0;
-            for (int m = 0; m < a.M; m++) s += cj[m] * mono[m][p];
-            basis[j][p] = s;
+            // four interleaved partial sums (fixed order) shorten the
+            // dependent FMA chain
+            double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+            int m = 0;
+            for (; m + 4 <= a.M; m += 4) {
+                s0 += cj[m] * mono[m][p];
+                s1 += cj[m + 1] * mono[m + 1][p];
+                s2 += cj[m + 2] * mono[m + 2][p];
+                s3 += cj[m + 3] * mono[m + 3][p];
+            }
+            for (; m < a.M; m++) s0 += cj[m] * mono[m][p];
+            basis[j][p] = (s0 + s1) + (s2 + s3);
         }
         __syncthreads();
 #pragma unroll
@@ -110,9 +119,15 @@ __global__ __launch_bounds__(kZThreads) void k_zernike_gram(ZArgs a)
             if (tid + k * kZThreads < a.nent) {
                 const double* br = basis[er[k]];
                 const double* bc = basis[ec[k]];
-                double s = 0.0;
-                for (int p = 0; p < kZChunk; p++) s += br[p] * bc[p];
-                acc[k] += s;
+                double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+#pragma unroll
+                for (int p = 0; p < kZChunk; p += 4) {
+                    s0 += br[p] * bc[p];
+                    s1 += br[p + 1] * bc[p + 1];
+                    s2 += br[p + 2] * bc[p + 2];
+                    s3 += br[p + 3] * bc[p + 3];
+                }
+                acc[k] += (s0 + s1) + (s2 + s3);
             }
         }
         __syncthreads();

@@ -11,6 +11,7 @@ import numpy as np
 from . import _lib, _device
 from .constants import globalCoordSys, vacuum
 from .obscuration import ObscNegation, ObscCircle, ObscAnnulus
+from .coating import SimpleCoating
 
 
 _DERIVED = ('path', 'itemDict', '_names', 'R_inner', 'R_outer', 'inRadius',
@@ -148,6 +149,34 @@ class Interface(Optic):
         if self.name not in unless:
             self.obscuration = None
 
+    def traceSplit(self, rv, minFlux=1e-3, reverse=False, _verbose=False):
+        """Split rays at this Interface into those continuing forward and
+        those sent back (optic.py:466-535); returns two lists of
+        RayVectors (None entries where a direction gets no rays), each
+        tagged with its ``path``."""
+        if _verbose:
+            s = "forward" if not reverse else "reverse"
+            print("traceSplit {}       {:30s} flux = {:18.8f}   nphot = {:10d}".format(
+                s, self.name, float(rv.torch("flux").sum()), len(rv)))
+        if self.skip:
+            return (None, rv) if reverse else (rv, None)
+        refracted, reflected = self.rSplit(rv, reverse=reverse)
+        # the obscuration applies to both directions
+        if self.obscuration is not None:
+            for r in (refracted, reflected):
+                if r is not None:
+                    self.obscuration.obscure(r)
+        path = list(getattr(rv, 'path', [])) + [self.name]
+        for r in (refracted, reflected):
+            if r is not None:
+                r.path = list(path)
+        rForward, rReverse = [refracted], [reflected]
+        if reverse:
+            rForward, rReverse = rReverse, rForward
+        if isinstance(self, Mirror):
+            rForward, rReverse = rReverse, rForward
+        return rForward, rReverse
+
     @cached_property
     def R_outer(self):
         obsc = self.obscuration
@@ -206,7 +235,21 @@ class Interface(Optic):
 
 class RefractiveInterface(Interface):
     """Refracts from inMedium into outMedium (reverse swaps them)
-    (optic.py:762-797)."""
+    (optic.py:762-797).  Coatings default to full transmission."""
+
+    def __init__(self, *args, **kwargs):
+        Interface.__init__(self, *args, **kwargs)
+        self.forwardCoating = SimpleCoating(reflectivity=0.0, transmissivity=1.0)
+        self.reverseCoating = SimpleCoating(reflectivity=0.0, transmissivity=1.0)
+
+    def rSplit(self, rv, reverse=False):
+        """(refracted, reflected); as optic.py:786-797, the forward direction
+        uses reverseCoating and the reverse direction forwardCoating."""
+        if reverse:
+            m1, m2, coating = self.outMedium, self.inMedium, self.forwardCoating
+        else:
+            m1, m2, coating = self.inMedium, self.outMedium, self.reverseCoating
+        return self.surface.rSplit(rv, m1, m2, coating, coordSys=self.coordSys)
 
     def _interact(self, rv, reverse, obscuration):
         from .trace import refract
@@ -220,6 +263,15 @@ class RefractiveInterface(Interface):
 class Mirror(Interface):
     """Reflects (optic.py:800-823)."""
 
+    def __init__(self, *args, **kwargs):
+        Interface.__init__(self, *args, **kwargs)
+        self.forwardCoating = SimpleCoating(reflectivity=1.0, transmissivity=0.0)
+        self.reverseCoating = SimpleCoating(reflectivity=1.0, transmissivity=0.0)
+
+    def rSplit(self, rv, reverse=False):
+        from .trace import reflect
+        return None, reflect(self.surface, rv, self.coordSys)
+
     def _interact(self, rv, reverse, obscuration):
         from .trace import reflect
         return reflect(self.surface, rv, self.coordSys, obscuration=obscuration)
@@ -231,9 +283,28 @@ class Mirror(Interface):
 class Detector(Interface):
     """Rays stop here (intersect only) (optic.py:826-845)."""
 
+    def __init__(self, *args, **kwargs):
+        Interface.__init__(self, *args, **kwargs)
+        self.forwardCoating = SimpleCoating(reflectivity=0.0, transmissivity=1.0)
+        self.reverseCoating = None
+
+    def rSplit(self, rv, reverse=False):
+        assert not reverse
+        return self.surface.rSplit(rv, self.inMedium, self.outMedium, self.forwardCoating,
+                                   coordSys=self.coordSys)
+
 
 class Baffle(Interface):
     """Rays pass straight through; only vignetting applies (optic.py:848-864)."""
+
+    def __init__(self, *args, **kwargs):
+        Interface.__init__(self, *args, **kwargs)
+        self.forwardCoating = SimpleCoating(reflectivity=0.0, transmissivity=1.0)
+        self.reverseCoating = SimpleCoating(reflectivity=0.0, transmissivity=1.0)
+
+    def rSplit(self, rv, reverse=False):
+        from .trace import intersect
+        return intersect(self.surface, rv, self.coordSys), None
 
 
 class OPDScreen(Interface):
@@ -245,6 +316,9 @@ class OPDScreen(Interface):
         self.screen = screen
 
     def _interact(self, rv, reverse, obscuration):
+        raise NotImplementedError("OPDScreen is outside the GPU trace path")
+
+    def rSplit(self, rv, reverse=False):
         raise NotImplementedError("OPDScreen is outside the GPU trace path")
 
 
@@ -350,43 +424,111 @@ class CompoundOptic(Optic):
         return rv
 
     def _trace_path(self, rv, path):
-        i = 0
-        nominalOrder = {}
-        for name in path:
-            if name not in nominalOrder:
-                nominalOrder[name] = i
-                i += 1
-        direction = "forward"
-        for i in range(len(path)):
-            currentName = path[i]
-            item = self[currentName]
-            if i == len(path) - 1:
-                nextDirection = "reverse" if nominalOrder[path[i]] == 0 else direction
-            else:
-                nextName = path[i + 1]
-                nextDirection = ("reverse" if nominalOrder[nextName] < nominalOrder[currentName]
-                                 else "forward")
-            if direction == nextDirection:
-                item.trace(rv, reverse=(direction == "reverse"))
-            else:
-                direction = nextDirection
-                from .trace import reflect
+        """Trace along ``path`` in place (optic.py:1052-1080)."""
+        from .trace import reflect
+        for item, turn, direction in self._path_steps(path):
+            if turn:
                 reflect(item.surface, rv, item.coordSys, obscuration=item.obscuration)
+            else:
+                item.trace(rv, reverse=(direction == "reverse"))
         return rv
 
     def traceFull(self, rv, reverse=False, path=None):
         """Per-Interface history {name: {'name', 'in', 'out'}}
-        (optic.py:1085-1182)."""
+        (optic.py:1085-1182).  With ``path`` the items are traced in that
+        order (a turn back reflects off the item's surface), keyed
+        ``name_0``, ``name_1``, ... for repeated visits."""
         from collections import OrderedDict
         from .plan import collect_steps
-        if path is not None:
-            raise NotImplementedError("traceFull with path is not supported")
         out = OrderedDict()
-        for iface, _, _ in collect_steps(self, reverse):
-            rin = rv.copy()
-            iface.trace(rv, reverse=reverse)
-            out[iface.name] = {'name': iface.name, 'in': rin, 'out': rv.copy()}
+        if path is None:
+            for iface, _, _ in collect_steps(self, reverse):
+                rin = rv.copy()
+                iface.trace(rv, reverse=reverse)
+                out[iface.name] = {'name': iface.name, 'in': rin, 'out': rv.copy()}
+            return out
+        from .trace import reflect
+        rv_in = rv
+        for item, turn, direction in self._path_steps(path):
+            rv_out = rv_in.copy()
+            if turn:
+                reflect(item.surface, rv_out, item.coordSys, obscuration=item.obscuration)
+            else:
+                item.trace(rv_out, reverse=(direction == "reverse"))
+            key = item.name + '_0'
+            j = 1
+            while key in out:
+                key = item.name + '_{}'.format(j)
+                j += 1
+            out[key] = {'name': item.name, 'in': rv_in.copy(), 'out': rv_out.copy()}
+            rv_in = rv_out
         return out
+
+    def _path_steps(self, path):
+        """(item, turns back here, direction after it) along ``path``
+        (optic.py:1052-1080): the direction flips where the next item comes
+        earlier in the nominal order than the current one."""
+        nominalOrder = {}
+        for name in path:
+            if name not in nominalOrder:
+                nominalOrder[name] = len(nominalOrder)
+        direction = "forward"
+        steps = []
+        for i, name in enumerate(path):
+            if i == len(path) - 1:
+                nextDirection = "reverse" if nominalOrder[name] == 0 else direction
+            else:
+                nextDirection = ("reverse" if nominalOrder[path[i + 1]] < nominalOrder[name]
+                                 else "forward")
+            turn = nextDirection != direction
+            direction = nextDirection
+            steps.append((self[name], turn, direction))
+        return steps
+
+    def traceSplit(self, rv, minFlux=1e-3, reverse=False, _verbose=False):
+        """Recursively split rays at every Interface into forward and reverse
+        parts (optic.py:1184-1282); rays that are vignetted or fall below
+        ``minFlux`` are dropped after each item (on the GPU: an
+        order-preserving compaction).  Returns (forward, reverse) lists of
+        RayVectors, each with its ``path``."""
+        if _verbose:
+            s = "reverse" if reverse else "forward"
+            print("traceSplit {}       {:30s} flux = {:18.8f}   nphot = {:10d}".format(
+                s, self.name, float(rv.torch("flux").sum()), len(rv)))
+        if self.skip:
+            return (None, rv) if reverse else (rv, None)
+        work = [(rv, "forward", 0)] if not reverse else [(rv, "reverse", len(self.items) - 1)]
+        outForward, outReverse = [], []
+        while work:
+            rays, direction, index = work.pop()
+            item = self.items[index]
+            rF, rR = item.traceSplit(rays, minFlux=minFlux, reverse=(direction == "reverse"),
+                                     _verbose=_verbose)
+            lists = []
+            for rl in (rF, rR):
+                kept = []
+                if not isinstance(rl, list):           # a skipped item passes rays on
+                    rl = [rl]
+                for rr in rl:
+                    if not rr:
+                        continue
+                    sel = rr._select(minFlux)
+                    sel.path = rr.path
+                    kept.append(sel)
+                lists.append(kept)
+            for rr in lists[0]:
+                if len(rr) > 0:
+                    if index == len(self.items) - 1:
+                        outForward.append(rr)
+                    else:
+                        work.append((rr, "forward", index + 1))
+            for rr in lists[1]:
+                if len(rr) > 0:
+                    if index == 0:
+                        outReverse.append(rr)
+                    else:
+                        work.append((rr, "reverse", index - 1))
+        return outForward, outReverse
 
     def clearObscuration(self, unless=()):
         for item in self.items:

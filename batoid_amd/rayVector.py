@@ -54,6 +54,34 @@ class RayVector:
         ret.coordSys = coordSys
         return ret
 
+    @staticmethod
+    def _empty(n, coordSys):
+        """n rays of uninitialised device columns."""
+        dev = _device.device()
+        cols = {k: torch.empty(int(n), dtype=torch.float64, device=dev) for k in _FCOLS}
+        for k in _BCOLS:
+            cols[k] = torch.empty(int(n), dtype=torch.uint8, device=dev)
+        return RayVector._directInit(cols, coordSys)
+
+    def _select(self, minFlux):
+        """The rays that are not vignetted and have flux >= minFlux, in order,
+        as a new RayVector (the filter of optic.py:1256-1266), by the
+        library's two-pass compaction."""
+        lib = _lib.load()
+        n = len(self)
+        ws = torch.empty(int(lib.bt_select_workspace_bytes(n)), dtype=torch.uint8,
+                         device=_device.device())
+        count = torch.zeros(1, dtype=torch.int64, device=ws.device)
+        stream = _device.stream()
+        _lib.call("bt_select_count", self._rays, float(minFlux), _device.ptr(count),
+                  _device.ptr(ws), ws.numel(), stream)
+        m = int(count.item())
+        out = RayVector._empty(m, self.coordSys)
+        if m:
+            _lib.call("bt_select_scatter", self._rays, float(minFlux), out._rays,
+                      _device.ptr(ws), ws.numel(), stream)
+        return out
+
     @classmethod
     def fromTorch(cls, x, y, z, vx, vy, vz, t, wavelength, flux, vignetted, failed,
                   coordSys=globalCoordSys):

@@ -4,6 +4,8 @@ Each call computes the CoordTransform from the rays' frame to the target
 frame on the host (coordTransform.py:16-20) and launches one fused HIP
 kernel (transform + intersect + interaction [+ obscuration]) over the rays,
 in place; the RayVector then reports the target frame."""
+import ctypes
+
 from . import _lib, _device
 from .coordTransform import CoordTransform
 
@@ -11,11 +13,6 @@ from .coordTransform import CoordTransform
 def _niter():
     from .global_vars import _batoid_niter
     return _batoid_niter
-
-
-def _no_coating(coating):
-    if coating is not None:
-        raise NotImplementedError("coatings are not part of the GPU trace path")
 
 
 def applyForwardTransform(ct, rv):
@@ -60,31 +57,40 @@ def _obsc_args(obscuration):
     return dev.ptr(), dev.count
 
 
+def _coating_ptr(coating):
+    """Host pointer to a bt_coating record (None: no coating); the record is
+    copied into the launch, so a temporary suffices."""
+    if coating is None:
+        return None
+    return ctypes.byref(coating._record)
+
+
 def intersect(surface, rv, coordSys=None, coating=None, obscuration=None):
     """Propagate rays to ``surface`` (in ``coordSys``) in place
-    (trace.py:47-81).  ``obscuration``, if given, is applied in the same pass."""
-    _no_coating(coating)
+    (trace.py:47-81); a coating scales the flux of the rays that reach it by
+    its transmissivity.  ``obscuration``, if given, is applied in the same
+    pass."""
     if coordSys is None:
         coordSys = rv.coordSys
     ct = CoordTransform(rv.coordSys, coordSys)
     optr, olen = _obsc_args(obscuration)
     _lib.call("bt_intersect", surface._dev.ptr(), _lib.dvec(ct.dr),
-              _lib.dvec(ct.drot.ravel()), rv._rays, _niter(), optr, olen,
-              _device.stream())
+              _lib.dvec(ct.drot.ravel()), rv._rays, _niter(), _coating_ptr(coating), optr,
+              olen, _device.stream())
     rv.coordSys = coordSys
     return rv
 
 
 def reflect(surface, rv, coordSys=None, coating=None, obscuration=None):
-    """Intersect and reflect rays in place (trace.py:84-98)."""
-    _no_coating(coating)
+    """Intersect and reflect rays in place (trace.py:84-98); a coating scales
+    the flux by its reflectivity."""
     if coordSys is None:
         coordSys = rv.coordSys
     ct = CoordTransform(rv.coordSys, coordSys)
     optr, olen = _obsc_args(obscuration)
     _lib.call("bt_reflect", surface._dev.ptr(), _lib.dvec(ct.dr),
-              _lib.dvec(ct.drot.ravel()), rv._rays, _niter(), optr, olen,
-              _device.stream())
+              _lib.dvec(ct.drot.ravel()), rv._rays, _niter(), _coating_ptr(coating), optr,
+              olen, _device.stream())
     rv.coordSys = coordSys
     return rv
 
@@ -92,21 +98,33 @@ def reflect(surface, rv, coordSys=None, coating=None, obscuration=None):
 def refract(surface, rv, m1, m2, coordSys=None, coating=None, obscuration=None):
     """Intersect and refract rays in place from ``m1`` into ``m2``
     (trace.py:101-116).  As in the reference kernel, the incoming index is
-    taken from |v|, so ``m1`` is not evaluated."""
-    _no_coating(coating)
+    taken from |v|, so ``m1`` is not evaluated.  A coating scales the flux by
+    its transmissivity."""
     if coordSys is None:
         coordSys = rv.coordSys
     ct = CoordTransform(rv.coordSys, coordSys)
     optr, olen = _obsc_args(obscuration)
     _lib.call("bt_refract", surface._dev.ptr(), _lib.dvec(ct.dr),
-              _lib.dvec(ct.drot.ravel()), m2._dev.ptr(), rv._rays, _niter(), optr, olen,
-              _device.stream())
+              _lib.dvec(ct.drot.ravel()), m2._dev.ptr(), rv._rays, _niter(),
+              _coating_ptr(coating), optr, olen, _device.stream())
     rv.coordSys = coordSys
     return rv
 
 
-def rSplit(*args, **kwargs):
-    raise NotImplementedError("rSplit (ghost splitting) is outside the GPU trace path")
+def rSplit(surface, rv, inMedium, outMedium, coating, coordSys=None):
+    """Split rays at ``surface`` into refracted (``rv``, in place) and
+    reflected (a new RayVector) parts with the coating's flux fractions
+    (trace.py:119-136, batoid.cpp:509-674).  Returns (refracted, reflected)."""
+    from .rayVector import RayVector
+    if coordSys is None:
+        coordSys = rv.coordSys
+    ct = CoordTransform(rv.coordSys, coordSys)
+    split = RayVector._empty(len(rv), coordSys)
+    _lib.call("bt_rsplit", surface._dev.ptr(), _lib.dvec(ct.dr), _lib.dvec(ct.drot.ravel()),
+              outMedium._dev.ptr(), _coating_ptr(coating), rv._rays, split._rays, _niter(),
+              _device.stream())
+    rv.coordSys = coordSys
+    return rv, split
 
 
 def refractScreen(*args, **kwargs):

@@ -261,6 +261,80 @@ def bench_raygen(optic, reps=20, cpu=True):
     return res
 
 
+def bench_zernike(optic, nx=512, jmax=37, reps=20, cpu=True):
+    """Zernike decomposition of the C2 wavefront: the TMA at field 0,
+    620 nm, nx x nx pupil grid, Noll j <= jmax.  Times the whole
+    batoid.zernike call and the bt_zernike_gram pass alone (HIP events),
+    and the reference's host fit (closed-form basis + numpy lstsq, as the
+    oracle restates it) on the same rays."""
+    import torch
+    import batoid_amd as bt
+    from batoid_amd import _lib, _device, workloads
+    from batoid_amd.analysis import _sphere_rays, zernike_table
+    thx, thy = workloads.tma_fields()
+    wl = 620e-9
+    bt.zernike(optic, thx[0], thy[0], wl, nx=nx, jmax=jmax)          # warm-up
+    torch.cuda.synchronize()
+    t0w = time.perf_counter()
+    coefs = bt.zernike(optic, thx[0], thy[0], wl, nx=nx, jmax=jmax)
+    torch.cuda.synchronize()
+    e2e = time.perf_counter() - t0w
+    # the Gram pass alone, on the same rays
+    dirCos = bt.utils.fieldToDirCos(thx[0], thy[0])
+    rays = bt.RayVector.asGrid(optic=optic, wavelength=wl, nx=nx, dirCos=dirCos)
+    ep = rays.toCoordSys(optic.stopSurface.coordSys)
+    optic.stopSurface.surface.intersect(ep)
+    srays, t0 = _sphere_rays(optic, thx[0], thy[0], wl, 'postel', nx, None, 'chief')
+    C, nmax = zernike_table(jmax)
+    coef = _device.f64(C)
+    lib = _lib.load()
+    ws = torch.empty(int(lib.bt_zernike_workspace_bytes(jmax)), dtype=torch.uint8,
+                     device=_device.device())
+    gram = torch.empty((jmax + 1) * (jmax + 2) // 2, dtype=torch.float64, device=ws.device)
+    stream = _device.stream()
+    args = (_device.ptr(ep.torch("x")), _device.ptr(ep.torch("y")), _device.ptr(srays.torch("t")),
+            None, _device.ptr(srays.torch("vignetted")), nx * nx, float(t0), wl,
+            optic.pupilSize / 2, _device.ptr(coef), jmax, nmax, _device.ptr(gram),
+            _device.ptr(ws), ws.numel(), stream)
+    _lib.check(lib.bt_zernike_gram(*args), "bt_zernike_gram")
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        lib.bt_zernike_gram(*args)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    M = C.shape[1]
+    flop_per_ray = 2.0 * (jmax * M + (jmax + 1) * (jmax + 2) // 2)
+    tflops = flop_per_ray * nx * nx / (ms * 1e-3) / 1e12
+    res = {"workload": f"TMA field 0, 620 nm, asGrid nx={nx}, Noll j<={jmax}",
+           "rays": nx * nx, "unvignetted": int((srays.torch("vignetted") == 0).sum().item()),
+           "end_to_end_ms": e2e * 1e3, "gram_kernel_ms": ms,
+           "fp64": {"achieved": tflops, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                    "frac": tflops / FP64_PEAK_TFLOPS, "flop_per_ray": flop_per_ray},
+           "z4_z11_waves": [float(coefs[4]), float(coefs[11])]}
+    log(f"zernike nx={nx} j<={jmax}: end to end {e2e * 1e3:.1f} ms, gram pass {ms:.3f} ms "
+        f"({tflops:.2f} TFLOP/s FP64)")
+    if cpu:
+        from oracle import zfit
+        x = ep.x
+        y = ep.y
+        w = (t0 - srays.t) / wl
+        keep = ~srays.vignetted
+        t1 = time.perf_counter()
+        basis = zfit.zernike_basis(jmax, x[keep], y[keep], R_outer=optic.pupilSize / 2)
+        np.linalg.lstsq(basis.T, w[keep], rcond=None)
+        t = time.perf_counter() - t1
+        res["cpu_baseline"] = {"value": t * 1e3, "unit": "ms per fit", "cores": 1,
+                               "kind": "port",
+                               "sample": "the same rays: closed-form basis + numpy lstsq "
+                                         "(the reference's galsim + lstsq step)"}
+        log(f"zernike cpu fit: {t * 1e3:.1f} ms")
+    return res
+
+
 def bench_newton_start(nrad, naz, optic_name="perturbed_tma", reps=5, train_steps=3000):
     """An optic with Newton mirrors traced with the reference start, the
     analytic base start ('conic': the mirror's conic root, or the ray origin
@@ -434,6 +508,7 @@ def main():
     ap.add_argument("--no-newton-start", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-raygen", action="store_true")
+    ap.add_argument("--no-zernike", action="store_true")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="also run a plain device copy of this many MiB (calibrates "
                          "FETCH_SIZE / WRITE_SIZE counters against known bytes)")
@@ -605,6 +680,8 @@ def main():
         result["huygens_nx640"] = bench_huygens(optic, nx=640)
     if not args.no_raygen:
         result["raygen"] = bench_raygen(optic, cpu=not args.no_cpu)
+    if not args.no_zernike:
+        result["zernike"] = bench_zernike(optic, cpu=not args.no_cpu)
     if not args.no_newton_start:
         result["newton_start"] = {
             name: bench_newton_start(args.nrad, args.naz, name)

@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define BT_ABI_VERSION 3
+#define BT_ABI_VERSION 4
 
 /* ---- status codes ------------------------------------------------------ */
 #define BT_OK            0
@@ -206,7 +206,7 @@ typedef struct bt_plan {
 int bt_abi_version(void);
 const char* bt_last_error(void);
 /* sizeof of bt_rays, bt_surface, bt_medium, bt_obsc, bt_step, bt_plan,
- * bt_psf_segment, bt_raygen, in that order, written to sizes[0..n-1]; lets a binding
+ * bt_psf_segment, bt_raygen, bt_coating, in that order, written to sizes[0..n-1]; lets a binding
  * check its layout. */
 int bt_struct_sizes(int64_t* sizes, int n);
 /* Device properties the host uses for launch sizing. */
@@ -288,23 +288,63 @@ typedef struct bt_raygen {
 
 int bt_generate_rays(const bt_raygen* gen, bt_rays out, void* stream);
 
+/* ---- coatings ----------------------------------------------------------
+ * SimpleCoating (src/coating.cpp:10-39), the reference's only coating:
+ * constant reflected and transmitted flux fractions.  Host struct, passed by
+ * pointer and copied into the launch.                                      */
+typedef struct bt_coating {
+    double reflectivity;
+    double transmissivity;
+} bt_coating;
+
 /* ---- per-surface trace --------------------------------------------------
  * Replace _batoid.intersect / reflect / refract (src/batoid.cpp:175-506,
- * pysrc/batoid.cpp:55-57) with coating = None and the caller's niter.
- * `surface` is a device pointer to the surface's first record; `m2` a device
- * pointer to the outgoing medium (m1 is not needed: the reference takes n1
- * from |v|, src/batoid.cpp:454-458).  `obsc`/`obsc_len` optionally fold the
- * Interface's obscuration into the same pass (optic.py:419-422); pass NULL/0
- * for the plain reference call.                                            */
+ * pysrc/batoid.cpp:55-57) with the caller's niter.  `surface` is a device
+ * pointer to the surface's first record; `m2` a device pointer to the
+ * outgoing medium (m1 is not needed: the reference takes n1 from |v|,
+ * src/batoid.cpp:454-458).  `coating` (host pointer, NULL = none)
+ * multiplies the flux of every ray that reaches the surface in this call by
+ * its transmissivity (intersect, refract -- total internal reflection
+ * included) or reflectivity (reflect), as src/batoid.cpp:251-263, 362-371,
+ * 497-499 do.  `obsc`/`obsc_len` optionally fold the Interface's
+ * obscuration into the same pass (optic.py:419-422); pass NULL/0 for the
+ * plain reference call.                                                    */
 int bt_intersect(const bt_surface* surface, const double* dr,
                  const double* drot, bt_rays rays, int niter,
-                 const bt_obsc* obsc, int obsc_len, void* stream);
+                 const bt_coating* coating, const bt_obsc* obsc, int obsc_len,
+                 void* stream);
 int bt_reflect(const bt_surface* surface, const double* dr,
                const double* drot, bt_rays rays, int niter,
-               const bt_obsc* obsc, int obsc_len, void* stream);
+               const bt_coating* coating, const bt_obsc* obsc, int obsc_len,
+               void* stream);
 int bt_refract(const bt_surface* surface, const double* dr,
                const double* drot, const bt_medium* m2, bt_rays rays,
-               int niter, const bt_obsc* obsc, int obsc_len, void* stream);
+               int niter, const bt_coating* coating, const bt_obsc* obsc,
+               int obsc_len, void* stream);
+/* Replaces _batoid.rSplit (src/batoid.cpp:509-674, pysrc/batoid.cpp:58):
+ * each ray that reaches the surface is split into the refracted ray
+ * (written back to `rays`, flux x transmissivity) and the reflected ray
+ * (written to `split`, flux x reflectivity).  Rays already failed on entry
+ * are copied to `split` unchanged (the reference's rvSplit starts as a copy
+ * of rv, batoid/trace.py:125); rays that miss the surface are
+ * marked failed and vignetted in both.  `split` must hold rays.n rays. */
+int bt_rsplit(const bt_surface* surface, const double* dr, const double* drot,
+              const bt_medium* m2, const bt_coating* coating, bt_rays rays,
+              bt_rays split, int niter, void* stream);
+
+/* ---- ray selection ------------------------------------------------------
+ * The ray filter of CompoundOptic.traceSplit (optic.py:1256-1266): keep the
+ * rays with vignetted == 0 and flux >= min_flux, in input order.  Two calls:
+ * bt_select_count writes the number kept to *count (a device int64) and the
+ * per-block offsets to the workspace; after the caller has read the count
+ * and sized `out`, bt_select_scatter (same `in`, `min_flux` and workspace)
+ * writes the kept rays.  Workspace: bt_select_workspace_bytes(in.n).      */
+int64_t bt_select_workspace_bytes(int64_t n);
+int bt_select_count(bt_rays in, double min_flux, int64_t* count, void* workspace,
+                    int64_t workspace_bytes, void* stream);
+int bt_select_scatter(bt_rays in, double min_flux, bt_rays out, const void* workspace,
+                      int64_t workspace_bytes, void* stream);
+
 /* Replaces _batoid.obscure (src/batoid.cpp:151-172, pysrc/batoid.cpp:59). */
 int bt_obscure(const bt_obsc* obsc, int obsc_len, bt_rays rays, void* stream);
 

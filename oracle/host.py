@@ -30,6 +30,10 @@ _ORACLE_SIGS = {
     "or_reflect": [_V, _V, _V, _R, _I, _I],
     "or_refract": [_V, _V, _V, _V, _R, _I, _I],
     "or_obscure": [_V, _I, _R, _I],
+    "or_intersect_coated": [_V, _V, _V, _D, _R, _I, _I],
+    "or_reflect_coated": [_V, _V, _V, _D, _R, _I, _I],
+    "or_refract_coated": [_V, _V, _V, _V, _D, _R, _I, _I],
+    "or_rsplit": [_V, _V, _V, _V, _D, _D, _R, _R, _I, _I],
     "or_trace_plan": [_V, _R, _I, _I],
     "or_position_at_time": [_R, _D, _V, _V, _V, _I],
     "or_propagate_in_place": [_R, _D, _I],
@@ -44,6 +48,8 @@ _REF_SIGS = {
     "ref_reflect": [_V, _V, _V, _R, _I, _I],
     "ref_refract": [_V, _V, _V, _V, _R, _I, _I],
     "ref_obscure": [_V, _I, _R, _I],
+    "ref_coated": [_I, _V, _V, _V, _V, _D, _D, _R, _I, _I],
+    "ref_rsplit": [_V, _V, _V, _V, _D, _D, _R, _R, _I, _I],
     "ref_sag_n": [_V, _V, _V, _V, _LL],
     "ref_normal_n": [_V, _V, _V, _V, _LL],
     "ref_get_n_n": [_V, _V, _V, _LL],

@@ -708,8 +708,11 @@ void or_apply_reverse_transform(const double* dr, const double* drot,
     double vz = rv.vx[i]*drot[2] + rv.vy[i]*drot[5] + rv.vz[i]*drot[8];    \
     double t = rv.t[i];
 
-void or_intersect(const bt_surface* s, const double* dr, const double* drot,
-                  bt_rays rv, int niter, int nthreads)
+/* coat: NULL, or the coating's flux factor for rays that reach the
+ * surface (batoid.cpp:251-263, 362-371, 497-499; SimpleCoating's factors
+ * are constants, coating.cpp:28-39). */
+static void intersect_c(const bt_surface* s, const double* dr, const double* drot,
+                        bt_rays rv, int niter, int nthreads, const double* coat)
 {                                                    /* batoid.cpp:221-269 */
     #pragma omp parallel for num_threads(nthreads)
     for (int64_t i = 0; i < rv.n; i++) {
@@ -725,6 +728,7 @@ void or_intersect(const bt_surface* s, const double* dr, const double* drot,
                 rv.x[i] = x; rv.y[i] = y; rv.z[i] = z;
                 rv.vx[i] = vx; rv.vy[i] = vy; rv.vz[i] = vz;
                 rv.t[i] = t;
+                if (coat) rv.flux[i] *= *coat;
             } else {
                 rv.failed[i] = 1;
                 rv.vignetted[i] = 1;
@@ -733,8 +737,20 @@ void or_intersect(const bt_surface* s, const double* dr, const double* drot,
     }
 }
 
-void or_reflect(const bt_surface* s, const double* dr, const double* drot,
-                bt_rays rv, int niter, int nthreads)
+void or_intersect(const bt_surface* s, const double* dr, const double* drot,
+                  bt_rays rv, int niter, int nthreads)
+{
+    intersect_c(s, dr, drot, rv, niter, nthreads, NULL);
+}
+
+void or_intersect_coated(const bt_surface* s, const double* dr, const double* drot,
+                         double transmissivity, bt_rays rv, int niter, int nthreads)
+{
+    intersect_c(s, dr, drot, rv, niter, nthreads, &transmissivity);
+}
+
+static void reflect_c(const bt_surface* s, const double* dr, const double* drot,
+                      bt_rays rv, int niter, int nthreads, const double* coat)
 {                                                    /* batoid.cpp:319-377 */
     #pragma omp parallel for num_threads(nthreads)
     for (int64_t i = 0; i < rv.n; i++) {
@@ -758,6 +774,7 @@ void or_reflect(const bt_surface* s, const double* dr, const double* drot,
                 rv.x[i] = x; rv.y[i] = y; rv.z[i] = z;
                 rv.vx[i] = vx; rv.vy[i] = vy; rv.vz[i] = vz;
                 rv.t[i] = t;
+                if (coat) rv.flux[i] *= *coat;
             } else {
                 rv.failed[i] = 1;
                 rv.vignetted[i] = 1;
@@ -766,8 +783,21 @@ void or_reflect(const bt_surface* s, const double* dr, const double* drot,
     }
 }
 
-void or_refract(const bt_surface* s, const double* dr, const double* drot,
-                const bt_medium* m2, bt_rays rv, int niter, int nthreads)
+void or_reflect(const bt_surface* s, const double* dr, const double* drot,
+                bt_rays rv, int niter, int nthreads)
+{
+    reflect_c(s, dr, drot, rv, niter, nthreads, NULL);
+}
+
+void or_reflect_coated(const bt_surface* s, const double* dr, const double* drot,
+                       double reflectivity, bt_rays rv, int niter, int nthreads)
+{
+    reflect_c(s, dr, drot, rv, niter, nthreads, &reflectivity);
+}
+
+static void refract_c(const bt_surface* s, const double* dr, const double* drot,
+                      const bt_medium* m2, bt_rays rv, int niter, int nthreads,
+                      const double* coat)
 {                                                    /* batoid.cpp:429-505 */
     #pragma omp parallel for num_threads(nthreads)
     for (int64_t i = 0; i < rv.n; i++) {
@@ -814,9 +844,96 @@ void or_refract(const bt_surface* s, const double* dr, const double* drot,
                     rv.vy[i] = ovy / n2;
                     rv.vz[i] = ovz / n2;
                 }
+                if (coat) rv.flux[i] *= *coat;
             } else {
                 rv.failed[i] = 1;
                 rv.vignetted[i] = 1;
+            }
+        }
+    }
+}
+
+void or_refract(const bt_surface* s, const double* dr, const double* drot,
+                const bt_medium* m2, bt_rays rv, int niter, int nthreads)
+{
+    refract_c(s, dr, drot, m2, rv, niter, nthreads, NULL);
+}
+
+void or_refract_coated(const bt_surface* s, const double* dr, const double* drot,
+                       const bt_medium* m2, double transmissivity, bt_rays rv, int niter,
+                       int nthreads)
+{
+    refract_c(s, dr, drot, m2, rv, niter, nthreads, &transmissivity);
+}
+
+void or_rsplit(const bt_surface* s, const double* dr, const double* drot,
+               const bt_medium* m2, double reflectivity, double transmissivity,
+               bt_rays rv, bt_rays rvs, int niter, int nthreads)
+{                                                    /* batoid.cpp:509-674 */
+    /* rvs holds a copy of rv on entry (trace.py:125); rays already failed
+     * are left as they are in both */
+    #pragma omp parallel for num_threads(nthreads)
+    for (int64_t i = 0; i < rv.n; i++) {
+        LOCAL_FRAME
+        if (!rv.failed[i]) {
+            double dt = 0.0;
+            int success = or_time_to_intersect(s, x, y, z, vx, vy, vz, &dt, niter);
+            if (success) {
+                x += vx * dt;
+                y += vy * dt;
+                z += vz * dt;
+                t += dt;
+                double n1 = vx*vx;
+                n1 += vy*vy;
+                n1 += vz*vz;
+                n1 = 1/sqrt(n1);
+                double nvx = vx*n1;
+                double nvy = vy*n1;
+                double nvz = vz*n1;
+                double nx, ny, nz;
+                or_normal(s, x, y, &nx, &ny, &nz);
+                double alpha = nvx*nx;
+                alpha += nvy*ny;
+                alpha += nvz*nz;
+                if (alpha > 0) {
+                    nx *= -1; ny *= -1; nz *= -1;
+                    alpha *= -1;
+                }
+                /* reflected part */
+                rvs.x[i] = x; rvs.y[i] = y; rvs.z[i] = z;
+                rvs.vx[i] = vx - 2*alpha*nx/n1;
+                rvs.vy[i] = vy - 2*alpha*ny/n1;
+                rvs.vz[i] = vz - 2*alpha*nz/n1;
+                rvs.t[i] = t;
+                rvs.wavelength[i] = rv.wavelength[i];
+                rvs.flux[i] = rv.flux[i]*reflectivity;
+                rvs.vignetted[i] = rv.vignetted[i];
+                rvs.failed[i] = rv.failed[i];
+                /* refracted part */
+                double n2 = or_get_n(m2, rv.wavelength[i]);
+                double eta = n1/n2;
+                double sinsqr = eta*eta*(1-alpha*alpha);
+                rv.x[i] = x; rv.y[i] = y; rv.z[i] = z;
+                rv.t[i] = t;
+                if (sinsqr > 1.0) {
+                    rv.vx[i] = NAN; rv.vy[i] = NAN; rv.vz[i] = NAN;
+                    rv.failed[i] = 1;
+                    rv.vignetted[i] = 1;
+                } else {
+                    double nfactor = eta*alpha + sqrt(1-sinsqr);
+                    double ovx = eta*nvx - nfactor*nx;
+                    double ovy = eta*nvy - nfactor*ny;
+                    double ovz = eta*nvz - nfactor*nz;
+                    rv.vx[i] = ovx / n2;
+                    rv.vy[i] = ovy / n2;
+                    rv.vz[i] = ovz / n2;
+                }
+                rv.flux[i] *= transmissivity;
+            } else {
+                rv.vignetted[i] = 1;
+                rv.failed[i] = 1;
+                rvs.vignetted[i] = 1;
+                rvs.failed[i] = 1;
             }
         }
     }

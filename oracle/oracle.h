@@ -57,6 +57,16 @@ void or_reflect(const bt_surface* s, const double* dr, const double* drot,
                 bt_rays rays, int niter, int nthreads);
 void or_refract(const bt_surface* s, const double* dr, const double* drot,
                 const bt_medium* m2, bt_rays rays, int niter, int nthreads);
+void or_intersect_coated(const bt_surface* s, const double* dr, const double* drot,
+                         double transmissivity, bt_rays rv, int niter, int nthreads);
+void or_reflect_coated(const bt_surface* s, const double* dr, const double* drot,
+                       double reflectivity, bt_rays rv, int niter, int nthreads);
+void or_refract_coated(const bt_surface* s, const double* dr, const double* drot,
+                       const bt_medium* m2, double transmissivity, bt_rays rv, int niter,
+                       int nthreads);
+void or_rsplit(const bt_surface* s, const double* dr, const double* drot,
+               const bt_medium* m2, double reflectivity, double transmissivity,
+               bt_rays rv, bt_rays rvs, int niter, int nthreads);
 void or_obscure(const bt_obsc* prog, int len, bt_rays rays, int nthreads);
 /* Optic.trace as the reference's Python drives it: for each step, the
  * interaction call and then (if any) the obscure call, each a full pass. */

@@ -26,6 +26,7 @@
 #include "medium.h"
 #include "obscuration.h"
 #include "rayVector.h"
+#include "coating.h"
 
 #include "../include/batoid_amd.h"
 
@@ -185,6 +186,39 @@ void ref_refract(const bt_surface* s, const double* dr, const double* drot,
     batoid::RayVector rv = make_rv(rays);
     // m1 is unused by the reference kernel (n1 comes from |v|).
     batoid::refract(*surf, v3(dr), m3(drot), *med, *med, rv, nullptr, nthreads, niter);
+}
+
+// action 0/1/2 = intersect / reflect / refract with a SimpleCoating.
+void ref_coated(int action, const bt_surface* s, const double* dr, const double* drot,
+                const bt_medium* m2, double reflectivity, double transmissivity,
+                bt_rays rays, int niter, int nthreads)
+{
+    Built b;
+    const batoid::Surface* surf = build_surface(s, b);
+    batoid::SimpleCoating coating(reflectivity, transmissivity);
+    batoid::RayVector rv = make_rv(rays);
+    if (action == 0) {
+        batoid::intersect(*surf, v3(dr), m3(drot), rv, &coating, nthreads, niter);
+    } else if (action == 1) {
+        batoid::reflect(*surf, v3(dr), m3(drot), rv, &coating, nthreads, niter);
+    } else {
+        auto med = build_medium(m2);
+        batoid::refract(*surf, v3(dr), m3(drot), *med, *med, rv, &coating, nthreads, niter);
+    }
+}
+
+// rSplit; `split` must hold a copy of `rays` on entry (trace.py:125).
+void ref_rsplit(const bt_surface* s, const double* dr, const double* drot,
+                const bt_medium* m2, double reflectivity, double transmissivity,
+                bt_rays rays, bt_rays split, int niter, int nthreads)
+{
+    Built b;
+    const batoid::Surface* surf = build_surface(s, b);
+    auto med = build_medium(m2);
+    batoid::SimpleCoating coating(reflectivity, transmissivity);
+    batoid::RayVector rv = make_rv(rays);
+    batoid::RayVector rvs = make_rv(split);
+    batoid::rSplit(*surf, v3(dr), m3(drot), *med, *med, coating, rv, rvs, nthreads, niter);
 }
 
 void ref_obscure(const bt_obsc* prog, int len, bt_rays rays, int nthreads)

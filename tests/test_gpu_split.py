@@ -1,0 +1,216 @@
+"""GPU ghost splitting: coated interactions, rSplit, the ray selection and
+traceSplit, against the oracle (bit-for-bit) and the reference's own
+traceSplit tests (tests/test_rsplit.py)."""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+import torch
+
+import batoid_amd as bt
+import oracle
+from oracle import split as OS
+from oracle.host import HostRays, host_records, nthreads
+from helpers import (surfaces, media, random_rays, transforms, assert_cols_identical,
+                     FCOLS, BCOLS)
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+N = 20000
+
+
+def _gpu(h, coordSys=bt.globalCoordSys):
+    rv = bt.RayVector(**{k: h[k] for k in FCOLS}, vignetted=h["vignetted"].astype(bool),
+                      failed=h["failed"].astype(bool))
+    rv.coordSys = coordSys
+    return rv
+
+
+def _host(rv):
+    torch.cuda.synchronize()
+    return {k: getattr(rv, k) for k in FCOLS + BCOLS}
+
+
+def _dv(a):
+    a = np.ascontiguousarray(a, dtype=float).ravel()
+    return (ctypes.c_double * len(a))(*a)
+
+
+def _frame(dr, drot):
+    return bt.CoordSys(origin=dr, rot=np.asarray(drot).reshape(3, 3))
+
+
+@pytest.mark.parametrize("name,surf", surfaces(), ids=[s[0] for s in surfaces()])
+def test_coated_and_rsplit_bit_exact(name, surf):
+    O = oracle.load()
+    tab = host_records(surf)
+    med = media()[2][1]
+    mtab = host_records(med)
+    coat = bt.SimpleCoating(0.0371, 0.9588)
+    for tname, (dr, drot) in transforms():
+        cs = _frame(dr, drot)
+        base = HostRays(**random_rays(N, seed=21, upward=0.3, spread=0.3))
+        args = (tab.ptr(), _dv(dr), _dv(drot))
+        # coated interactions
+        for action in ("intersect", "reflect", "refract"):
+            ref = base.copy()
+            if action == "refract":
+                O.or_refract_coated(*args, mtab.ptr(), coat.transmissivity, ref.struct, 5,
+                                    nthreads())
+                rv = surf.refract(_gpu(base), bt.ConstMedium(1.0), med, coordSys=cs,
+                                  coating=coat)
+            elif action == "reflect":
+                O.or_reflect_coated(*args, coat.reflectivity, ref.struct, 5, nthreads())
+                rv = surf.reflect(_gpu(base), coordSys=cs, coating=coat)
+            else:
+                O.or_intersect_coated(*args, coat.transmissivity, ref.struct, 5, nthreads())
+                rv = surf.intersect(_gpu(base), coordSys=cs, coating=coat)
+            assert_cols_identical(_host(rv), ref.cols, what=f"{name}/{action}/{tname}")
+        # rSplit
+        a, sa = base.copy(), base.copy()
+        O.or_rsplit(*args, mtab.ptr(), coat.reflectivity, coat.transmissivity, a.struct,
+                    sa.struct, 5, nthreads())
+        refr, refl = surf.rSplit(_gpu(base), bt.ConstMedium(1.0), med, coat, coordSys=cs)
+        assert_cols_identical(_host(refr), a.cols, what=f"{name}/rsplit refracted/{tname}")
+        assert_cols_identical(_host(refl), sa.cols, what=f"{name}/rsplit reflected/{tname}")
+        assert refr.coordSys == cs and refl.coordSys == cs
+
+
+@pytest.mark.parametrize("n", [0, 1, 255, 256, 257, 100_003, 2_000_000])
+def test_select_matches_numpy(n):
+    rng = np.random.default_rng(n)
+    h = HostRays(**random_rays(max(n, 1), seed=n % 97))
+    if n == 0:
+        rv = bt.RayVector._empty(0, bt.globalCoordSys)
+        assert len(rv._select(1e-3)) == 0
+        return
+    flux = rng.uniform(0, 2e-3, n)
+    flux[rng.uniform(size=n) < 0.01] = np.nan
+    h.cols["flux"] = flux
+    h.cols["vignetted"] = (rng.uniform(size=n) < 0.3).astype(np.uint8)
+    rv = _gpu(h)
+    for mf in (1e-3, 0.0, 5e-3):
+        out = rv._select(mf)
+        keep = (h["vignetted"] == 0) & (h["flux"] >= mf)
+        assert len(out) == int(keep.sum())
+        got = _host(out)
+        for k in FCOLS + BCOLS:
+            np.testing.assert_array_equal(got[k], h[k][keep].astype(got[k].dtype), err_msg=k)
+
+
+def test_rsplit_reference_test():
+    """tests/test_rsplit.py:8-34: rSplit equals reflect / refract with the
+    same coating (up to the rounding of the reflection formula)."""
+    rng = np.random.default_rng(5)
+    for _ in range(30):
+        R = rng.normal(0.7, 0.8)
+        conic = rng.uniform(-2.0, 1.0)
+        ncoef = int(rng.integers(0, 5))
+        coefs = [rng.normal(0, 1e-10) for _ in range(ncoef)]
+        asphere = bt.Asphere(R, conic, coefs)
+        rays = bt.RayVector.asPolar(backDist=10.0, medium=bt.Air(), wavelength=500e-9,
+                                    outer=0.25 * R, theta_x=rng.normal(0.0, 1e-8),
+                                    theta_y=rng.normal(0.0, 1e-8), nrad=10, naz=10)
+        coating = bt.SimpleCoating(0.9, 0.1)
+        reflected = asphere.reflect(rays.copy(), coating=coating)
+        m1, m2 = bt.Air(), bt.ConstMedium(1.1)
+        refracted = asphere.refract(rays.copy(), m1, m2, coating=coating)
+        refracted2, reflected2 = asphere.rSplit(rays, m1, m2, coating)
+        for a, b in ((reflected, reflected2), (refracted, refracted2)):
+            for k in ("x", "y", "z", "vx", "vy", "vz", "t", "flux"):
+                np.testing.assert_allclose(getattr(a, k), getattr(b, k), rtol=0, atol=1e-14,
+                                           equal_nan=True)
+            np.testing.assert_array_equal(a.vignetted, b.vignetted)
+
+
+def _simple_telescope():
+    """tests/test_rsplit.py:38-80."""
+    telescope = bt.CompoundOptic(
+        name="simple", stopSurface=bt.Interface(bt.Plane()),
+        items=[
+            bt.Lens(name="L1", items=[
+                bt.RefractiveInterface(bt.Plane(), name="L1_entrance",
+                                       coordSys=bt.CoordSys(origin=[0, 0, 0.3]),
+                                       inMedium=bt.ConstMedium(1.0),
+                                       outMedium=bt.ConstMedium(1.1)),
+                bt.RefractiveInterface(bt.Plane(), name="L1_exit",
+                                       coordSys=bt.CoordSys(origin=[0, 0, 0.2]),
+                                       inMedium=bt.ConstMedium(1.1),
+                                       outMedium=bt.ConstMedium(1.0))]),
+            bt.Mirror(bt.Plane(), name="Mirror"),
+            bt.Detector(bt.Plane(), name="detector",
+                        coordSys=bt.CoordSys(origin=[0, 0, 0.1]))],
+        pupilSize=1.0, backDist=1.0, inMedium=bt.ConstMedium(1.0))
+    for name in ("L1_entrance", "L1_exit"):
+        telescope[name].forwardCoating = bt.SimpleCoating(0.02, 0.98)
+        telescope[name].reverseCoating = bt.SimpleCoating(0.02, 0.98)
+    telescope['detector'].forwardCoating = bt.SimpleCoating(0.02, 0.98)
+    return telescope
+
+
+def _check_paths(optic, rays, rF, rR, tol):
+    """Every traceSplit output equals trace(path=) and the last entry of
+    traceFull(path=) on its unvignetted rays (tests/test_rsplit.py:89-134)."""
+    for r in list(rF) + list(rR):
+        r2 = optic.trace(rays.copy(), path=r.path)
+        w = ~r2.vignetted
+        np.testing.assert_allclose(r.r, r2.r[w], atol=tol, rtol=0)
+        np.testing.assert_allclose(r.v, r2.v[w], atol=tol, rtol=0)
+        np.testing.assert_allclose(r.t, r2.t[w], atol=tol, rtol=0)
+        tf = optic.traceFull(rays.copy(), path=r.path)
+        keys = []
+        for item in r.path:
+            j = 0
+            key = f"{item}_{j}"
+            while key in keys:
+                j += 1
+                key = f"{item}_{j}"
+            keys.append(key)
+        assert keys == list(tf.keys())
+        r3 = tf[keys[-1]]['out']
+        w = ~r3.vignetted
+        np.testing.assert_allclose(r.r, r3.r[w], atol=tol, rtol=0)
+        np.testing.assert_allclose(r.v, r3.v[w], atol=tol, rtol=0)
+        np.testing.assert_allclose(r.t, r3.t[w], atol=tol, rtol=0)
+
+
+def _check_oracle(optic, rays, rF, rR, minFlux):
+    """The same split through the oracle's restatement: the same paths in
+    the same order, every column bit-identical."""
+    oF, oR = OS.trace_split(optic, OS.Rays(HostRays.from_rv(rays), rays.coordSys), minFlux)
+    for got, want in ((rF, oF), (rR, oR)):
+        assert [g.path for g in got] == [w.path for w in want]
+        for g, w in zip(got, want):
+            assert g.coordSys == w.coordSys
+            assert_cols_identical(_host(g), w.h.cols, what="/".join(g.path))
+
+
+def test_traceSplit_simple():
+    telescope = _simple_telescope()
+    rays = bt.RayVector.asPolar(optic=telescope, wavelength=500e-9, theta_x=np.deg2rad(1.0),
+                                theta_y=0.0, nrad=10, naz=60)
+    rF, rR = telescope.traceSplit(rays.copy(), minFlux=1e-4)
+    assert len(rF) > 1 and len(rR) > 1
+    _check_paths(telescope, rays, rF, rR, 1e-14)
+    _check_oracle(telescope, rays, rF, rR, 1e-4)
+
+
+@pytest.mark.parametrize("name", ["DECam.yaml", "HSC.yaml", "LSST_r.yaml"])
+def test_traceSplit_golden_optics(name):
+    """tests/test_rsplit.py:137-213: refractive interfaces and the detector
+    made 2 % reflective."""
+    optic = bt.Optic.fromYaml(os.path.join(GOLD, "optics", name))
+    for item in optic.itemDict.values():
+        if isinstance(item, bt.RefractiveInterface):
+            item.forwardCoating = bt.SimpleCoating(0.02, 0.98)
+            item.reverseCoating = bt.SimpleCoating(0.02, 0.98)
+        if isinstance(item, bt.Detector):
+            item.forwardCoating = bt.SimpleCoating(0.02, 0.98)
+    rays = bt.RayVector.asPolar(optic, wavelength=620e-9, theta_x=np.deg2rad(0.1),
+                                theta_y=0.0, nrad=15, naz=90)
+    minFlux = 1e-3
+    rF, rR = optic.traceSplit(rays.copy(), minFlux=minFlux)
+    assert len(rF) >= 1 and len(rF) + len(rR) > 2
+    _check_paths(optic, rays, rF, rR, 1e-12)
+    _check_oracle(optic, rays, rF, rR, minFlux)

@@ -49,6 +49,38 @@ def test_surface_interactions(name, surf, action):
 
 
 @pytest.mark.parametrize("name,surf", surfaces(), ids=[s[0] for s in surfaces()])
+def test_coated_interactions_and_rsplit(name, surf):
+    """Coated intersect / reflect / refract and rSplit (batoid.cpp:175-674
+    with a SimpleCoating) against the reference, every surface, two frames,
+    failures, total internal reflection and pre-failed rays included."""
+    O, R = oracle.load(), oracle.load_ref()
+    tab = host_records(surf)
+    med = host_records(media()[2][1])
+    refl, trans = 0.0371, 0.9588
+    for tname, (dr, drot) in transforms()[:2]:
+        base = HostRays(**random_rays(N, seed=9, upward=0.3, spread=0.3))
+        args = (tab.ptr(), _dv(dr), _dv(drot))
+        for action, coef in ((0, trans), (1, refl), (2, trans)):
+            a, b = base.copy(), base.copy()
+            if action == 2:
+                O.or_refract_coated(*args, med.ptr(), coef, a.struct, NITER, nthreads())
+            else:
+                getattr(O, "or_intersect_coated" if action == 0 else "or_reflect_coated")(
+                    *args, coef, a.struct, NITER, nthreads())
+            R.ref_coated(action, *args, med.ptr(), refl, trans, b.struct, NITER, 1)
+            assert_cols_identical(a, b, what=f"{name}/coated{action}/{tname}")
+        a, b = base.copy(), base.copy()
+        sa, sb = base.copy(), base.copy()
+        O.or_rsplit(*args, med.ptr(), refl, trans, a.struct, sa.struct, NITER, nthreads())
+        R.ref_rsplit(*args, med.ptr(), refl, trans, b.struct, sb.struct, NITER, 1)
+        assert_cols_identical(a, b, what=f"{name}/rsplit refracted/{tname}")
+        assert_cols_identical(sa, sb, what=f"{name}/rsplit reflected/{tname}")
+        ok = (base["failed"] == 0) & (a["failed"] == 0)
+        assert ok.sum() > 0
+        np.testing.assert_array_equal(a["flux"][ok], base["flux"][ok] * trans)
+
+
+@pytest.mark.parametrize("name,surf", surfaces(), ids=[s[0] for s in surfaces()])
 def test_sag_normal(name, surf):
     O, R = oracle.load(), oracle.load_ref()
     tab = host_records(surf)
