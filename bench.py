@@ -335,6 +335,78 @@ def bench_zernike(optic, nx=512, jmax=37, reps=20, cpu=True):
     return res
 
 
+def bench_trace_split(optic_factory, nrad=354, naz=3503, reps=10, cpu=True):
+    """Ghost splitting: the TMA with every refractive interface and the
+    detector 2 % reflective (the reference's traceSplit test setup),
+    traceSplit of a ~1e6-ray hexapolar pupil (minFlux 1e-4), and the
+    bt_rsplit kernel alone on the 9,998,544-ray C2 pupil at L1's entrance
+    (206 B of HBM traffic per ray: read 74, write 58 + 74)."""
+    import torch
+    import batoid_amd as bt
+    from batoid_amd import _device, workloads
+    optic = optic_factory()
+    for item in optic.itemDict.values():
+        if isinstance(item, bt.RefractiveInterface):
+            item.forwardCoating = bt.SimpleCoating(0.02, 0.98)
+            item.reverseCoating = bt.SimpleCoating(0.02, 0.98)
+        if isinstance(item, bt.Detector):
+            item.forwardCoating = bt.SimpleCoating(0.02, 0.98)
+    thx, thy = workloads.tma_fields()
+    rays = workloads.tma_rays(optic, thx[0], thy[0], 620e-9, nrad=nrad, naz=naz)
+    optic.traceSplit(rays.copy(), minFlux=1e-4)               # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rF, rR = optic.traceSplit(rays.copy(), minFlux=1e-4)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3
+    res = {"workload": f"TMA, 2 % reflective glass and detector, {len(rays)} rays, "
+                       "minFlux 1e-4",
+           "rays": len(rays), "ms_per_traceSplit": ms,
+           "forward_paths": len(rF), "reverse_paths": len(rR),
+           "rays_out": int(sum(len(r) for r in rF) + sum(len(r) for r in rR))}
+    log(f"traceSplit {len(rays)} rays: {ms:.1f} ms, {len(rF)} forward / {len(rR)} reverse "
+        f"paths")
+    # the rSplit kernel alone, C2 pupil at L1's entrance
+    big = workloads.tma_rays(optic, thx[0], thy[0], 620e-9)
+    l1 = optic['L1_entrance']
+    coat = l1.reverseCoating
+    work = big.copy()
+    bt.rSplit(l1.surface, work, l1.inMedium, l1.outMedium, coat, coordSys=l1.coordSys)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    times = []
+    for _ in range(reps):
+        work = big.copy()
+        torch.cuda.synchronize()
+        e0.record()
+        bt.rSplit(l1.surface, work, l1.inMedium, l1.outMedium, coat, coordSys=l1.coordSys)
+        e1.record()
+        torch.cuda.synchronize()
+        times.append(e0.elapsed_time(e1))
+    kms = float(np.median(times))
+    gbs = 206.0 * len(big) / (kms * 1e-3) / 1e9
+    res["rsplit_kernel"] = {"rays": len(big), "ms": kms,
+                            "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS,
+                                         "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+                                         "bytes_per_ray": 206}}
+    log(f"rSplit kernel {len(big)} rays: {kms:.3f} ms ({gbs:.0f} GB/s incl. allocation)")
+    if cpu:
+        from oracle import split as OS
+        from oracle.host import HostRays
+        small = workloads.tma_rays(optic, thx[0], thy[0], 620e-9, nrad=112, naz=1108)
+        h = OS.Rays(HostRays.from_rv(small), small.coordSys)
+        t1 = time.perf_counter()
+        OS.trace_split(optic, h, 1e-4)
+        t = time.perf_counter() - t1
+        res["cpu_baseline"] = {"value": len(small) / t, "unit": "input rays/s", "cores": 1,
+                               "kind": "port",
+                               "sample": f"{len(small)} rays through the oracle's traceSplit "
+                                         f"restatement, {t:.2f} s"}
+        log(f"traceSplit cpu: {len(small)} rays in {t:.2f} s")
+    return res
+
+
 def bench_newton_start(nrad, naz, optic_name="perturbed_tma", reps=5, train_steps=3000):
     """An optic with Newton mirrors traced with the reference start, the
     analytic base start ('conic': the mirror's conic root, or the ray origin
@@ -509,6 +581,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-raygen", action="store_true")
     ap.add_argument("--no-zernike", action="store_true")
+    ap.add_argument("--no-split", action="store_true")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="also run a plain device copy of this many MiB (calibrates "
                          "FETCH_SIZE / WRITE_SIZE counters against known bytes)")
@@ -682,6 +755,8 @@ def main():
         result["raygen"] = bench_raygen(optic, cpu=not args.no_cpu)
     if not args.no_zernike:
         result["zernike"] = bench_zernike(optic, cpu=not args.no_cpu)
+    if not args.no_split:
+        result["trace_split"] = bench_trace_split(workloads.tma, cpu=not args.no_cpu)
     if not args.no_newton_start:
         result["newton_start"] = {
             name: bench_newton_start(args.nrad, args.naz, name)
