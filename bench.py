@@ -36,8 +36,9 @@ BYTES_PER_RAY = 8 * 8 + 2 + 7 * 8 + 2
 # profiles/trace_plan_tma_1e7_counters.md.
 MEASURED_TRAFFIC_PER_RAY = 124.04
 # FP64 flops per ray of the same kernel on this workload: 64 lanes x
-# (ADD + MUL + 2 FMA) FP64 wave instructions (SQ_INSTS_VALU_*_F64), same file.
-FP64_FLOP_PER_RAY = 7024.0
+# (ADD + MUL + 2 FMA) FP64 wave instructions (SQ_INSTS_VALU_*_F64), same file
+# (current build's figure).
+FP64_FLOP_PER_RAY = 6942.0
 # Ray generation writes 9 FP64 columns and 2 flag bytes per ray.
 RAYGEN_BYTES_PER_RAY = 9 * 8 + 2
 
