@@ -27,8 +27,13 @@
 // (ray, re|im), with A'[2i][2r] = Re U, A'[2i][2r+1] = -Im U,
 // A'[2i+1][2r] = Im U, A'[2i+1][2r+1] = Re U, B'[2r][j] = Re V,
 // B'[2r+1][j] = Im V.  Operands are generated directly in the MFMA lane
-// layout; each wave owns a 64 (i) x 64 (j) complex output tile (eight
-// 32 x 32 accumulator blocks) and a contiguous range of rays.
+// layout.  A workgroup owns a 128 x 128 complex output tile and a contiguous
+// range of rays: FP32 as two waves of 64 (i) x 128 (j) (sixteen 32 x 32
+// accumulator blocks each), split-BF16 as four waves of 64 x 64 (eight).
+// On gfx950 the FP32 MFMA does not co-execute with VALU work, so the FP32
+// path spends as few VALU cycles per MFMA as it can: per ray and wave it
+// evaluates one sine/cosine pair for the rows and one for the columns and
+// derives the other blocks by rotations precomputed per ray in FP64.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <string>
@@ -41,14 +46,20 @@ extern "C" int64_t bt_huygens_direct_workspace_bytes(int64_t n, int nx, int ny);
 namespace bt {
 
 typedef float v16f __attribute__((ext_vector_type(16)));
+typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+typedef unsigned int v2u __attribute__((ext_vector_type(2)));
+typedef float v2f __attribute__((ext_vector_type(2)));
+typedef __bf16 v2bf __attribute__((ext_vector_type(2)));
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 
 constexpr double kTwoPi = 2 * 3.14159265358979323846;
 constexpr int kTile = 128;          // workgroup tile (complex) in i and j
 constexpr int kRayStride = 8;       // floats per prepared ray
-constexpr int kRotStride = 8;       // floats per ray of row-block rotations
+constexpr int kRotStride = 16;      // floats per ray of row/column-block rotations
 constexpr int kRayGroup = 8;        // rays per bf16 MFMA step; splits align to it
 constexpr int kScanThreads = 1024;
+// buffer resource word 3 for raw 32-bit loads (gfx9 family)
+constexpr int kBufferFlags = 0x00020000;
 
 struct PsfGeom {
     double p0x, p0y, a1x, a1y, a2x, a2y, t;
@@ -112,8 +123,10 @@ __global__ __launch_bounds__(kScanThreads) void k_psf_scan(const int* __restrict
 // Per ray: phase at p0 (revolutions) and per-pixel phase steps
 //   o   = [alpha_hi, alpha_lo, beta_hi, beta_lo, phi0_rev, flux, 0, 0]
 // and the rotations between the row blocks of a wave's tile (16, 32, 48
-// rows apart), with which the split-BF16 path derives blocks 1-3 from 0
-//   rot = [cos, sin of 2 pi frac(16 k alpha) for k = 1, 2, 3, 0, 0].
+// pixels apart) and between its column blocks (32, 64, 96 pixels apart),
+// with which blocks 1-3 are derived from block 0
+//   rot = [cos, sin of 2 pi frac(16 k alpha) for k = 1, 2, 3, 0, 0,
+//          cos, sin of 2 pi frac(32 k beta) for k = 1, 2, 3, 0, 0].
 __device__ __forceinline__ void prep_ray(const bt_rays& rv, int64_t i, double p0x, double p0y,
                                          double tref, double a1x, double a1y, double a2x,
                                          double a2y, int keep_a, int keep_b,
@@ -146,14 +159,24 @@ __device__ __forceinline__ void prep_ray(const bt_rays& rv, int64_t i, double p0
     o[5] = (float)rv.flux[i];
     o[6] = 0.f;
     o[7] = 0.f;
-    for (int k = 1; k <= 3; k++) {
-        double d = 16.0 * k * al;
-        d -= floor(d);
-        rot[2*k - 2] = (float)cos(kTwoPi * d);
-        rot[2*k - 1] = (float)sin(kTwoPi * d);
-    }
+    // one FP64 sincos per axis; the 2- and 3-block rotations by products
+    // (relative error ~1e-16, far below the FP32 rounding of the table)
+    auto blocks = [](double step, float* o) {
+        double d = step - floor(step);
+        double s1, c1;
+        sincos(kTwoPi * d, &s1, &c1);
+        const double c2 = c1 * c1 - s1 * s1, s2 = 2.0 * s1 * c1;
+        const double c3 = c2 * c1 - s2 * s1, s3 = s2 * c1 + c2 * s1;
+        o[0] = (float)c1; o[1] = (float)s1;
+        o[2] = (float)c2; o[3] = (float)s2;
+        o[4] = (float)c3; o[5] = (float)s3;
+    };
+    blocks(16.0 * al, rot);
+    blocks(32.0 * be, rot + 8);
     rot[6] = 0.f;
     rot[7] = 0.f;
+    rot[14] = 0.f;
+    rot[15] = 0.f;
 }
 
 // The parameters of every valid ray, compacted in ray order.
@@ -181,11 +204,30 @@ __global__ __launch_bounds__(kBlock) void k_psf_prep(bt_rays rv, PsfGeom g, int 
 __device__ __forceinline__ float sin_rev(float th) { return __builtin_amdgcn_sinf(th); }
 __device__ __forceinline__ float cos_rev(float th) { return __builtin_amdgcn_cosf(th); }
 
+// x = hi + lo for a pair (x0, x1): hi and lo as dwords of two bf16 (element
+// 0 in the low half), each rounded to nearest even by v_cvt_pk_bf16_f32.
+__device__ __forceinline__ void split_pair(float x0, float x1, uint32_t& hi, uint32_t& lo)
+{
+    hi = __builtin_bit_cast(uint32_t, __builtin_convertvector((v2f){x0, x1}, v2bf));
+    const float r0 = x0 - __uint_as_float(hi << 16);
+    const float r1 = x1 - __uint_as_float(hi & 0xffff0000u);
+    lo = __builtin_bit_cast(uint32_t, __builtin_convertvector((v2f){r0, r1}, v2bf));
+}
+
 __device__ __forceinline__ float step_phase(float c, float hi, float lo, float add)
 {
     // fract(c*hi) is exact: c*hi is exact by construction of hi
     return __builtin_amdgcn_fractf(c * hi) + __builtin_fmaf(c, lo, add);
 }
+
+// Threads per workgroup of the GEMM kernels: two waves (FP32) or four
+// (split-BF16) over the 128 x 128 tile, and the waves per SIMD each is
+// compiled for: the FP32 wave's 256 accumulators need the whole register
+// file (one wave per SIMD), split-BF16 runs two.
+template <int kPrec>
+constexpr int gemm_threads() { return kPrec == BT_PSF_FP32 ? 128 : 256; }
+template <int kPrec>
+constexpr int gemm_waves_per_simd() { return kPrec == BT_PSF_FP32 ? 1 : 2; }
 
 // One workgroup's 128 x 128 complex tile of the sum over rays [r0, r1) of
 // the prepared parameters, written as FP32 (re, im) into out[i][j][2] (the
@@ -196,119 +238,218 @@ __device__ __forceinline__ void psf_tile(const float* __restrict__ params,
                                          int tile_i0, int tile_j0, int nx, int ny, int nyp,
                                          float* __restrict__ out)
 {
+    // column blocks of 32 per wave: 4 (FP32, 64 x 128) or 2 (split-BF16, 64 x 64)
+    constexpr int CB = kPrec == BT_PSF_FP32 ? 4 : 2;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int i0 = tile_i0 + (wave & 1) * 64;
     const int j0 = tile_j0 + (wave >> 1) * 64;
     const int half = lane >> 5;       // k half of the operand
     const int l32 = lane & 31;
-    v16f acc[4][2];
+    v16f acc[4][CB];
     for (int a = 0; a < 4; a++)
-        for (int b = 0; b < 2; b++)
+        for (int b = 0; b < CB; b++)
             for (int e = 0; e < 16; e++) acc[a][b][e] = 0.f;
 
     // per-lane pixel offsets of the A rows (i) and B columns (j)
-    float ci[4], cj[2];
+    float ci[4], cj[CB];
     const int prow = l32 & 1;                          // re / im row
     for (int rb = 0; rb < 4; rb++) ci[rb] = (float)(i0 + rb*16 + (l32 >> 1) - nx/2);
-    for (int cb = 0; cb < 2; cb++) cj[cb] = (float)(j0 + cb*32 + l32 - ny/2);
+    for (int cb = 0; cb < CB; cb++) cj[cb] = (float)(j0 + cb*32 + l32 - ny/2);
 
-    if (kPrec == BT_PSF_FP32) {
+    if constexpr (kPrec == BT_PSF_FP32) {
         // 32x32x2: lane supplies A'[l32][half], B'[half][l32]; one ray per MFMA
         //   (prow, half) = (0,0): Re U = cos = sin(th + 1/4)
         //                  (0,1): -Im U = -sin = sin(th + 1/2)
         //                  (1,0): Im U = sin(th);  (1,1): Re U = sin(th + 1/4)
         // (th in revolutions), so every A value is one sine with a per-lane
-        // offset; the ray's flux scales its B values (Re V = cos, Im V = sin).
+        // offset, and so is every B value (Re V = cos, Im V = sin).  Block 0
+        // is evaluated as s = sin, c = cos of its phase; block k is then
+        // sin(th + k d) = s cos(k d) + c sin(k d) with the ray's rotations.
+        // The flux scales the A values.
         const float oa = (prow == half) ? 0.25f : (half == 1 ? 0.5f : 0.f);
         const float ob = (half == 0) ? 0.25f : 0.f;
-        // rays in groups of 4 whose parameters (wave-uniform) are fetched one
-        // group ahead of their use
-        constexpr int G = 4;
-        float cur[G][6], nxt[G][6];
-        auto fetch = [&](float (&dst)[G][6], int rbase) {
-            for (int q = 0; q < G; q++) {
-                const int rr = rbase + q;
-                const bool live = rr < r1;
-                const float* p = params + (int64_t)(live ? rr : r0) * kRayStride;
-                for (int e = 0; e < 5; e++) dst[q][e] = p[e];
-                dst[q][5] = live ? p[5] : 0.f;
+        // A ray's parameters (wave-uniform) as buffer loads into VGPRs, only
+        // the 18 dwords used: [ah, al, bh, bl], [phi0, flux], row rotations
+        // [c1, s1, c2, s2], [c3, s3], column rotations likewise.  Vector loads
+        // complete in order, so several rays can be in flight and waited for
+        // one at a time.  A ray at or past r1 gets a zero-sized buffer: its
+        // loads return zeros, so its flux -- and its contribution -- is zero.
+        struct RayRegs { v4u a; v2u b; v4u c; v2u d; v4u e; v2u f; };
+        auto load = [&](RayRegs& d, int rr) {
+            const bool live = rr < r1;
+            const int q = live ? rr : r0;
+            const __amdgpu_buffer_rsrc_t rp = __builtin_amdgcn_make_buffer_rsrc(
+                (void*)(params + (int64_t)q * kRayStride), (short)0,
+                live ? kRayStride * 4 : 0, kBufferFlags);
+            const __amdgpu_buffer_rsrc_t rq = __builtin_amdgcn_make_buffer_rsrc(
+                (void*)(rot + (int64_t)q * kRotStride), (short)0,
+                live ? kRotStride * 4 : 0, kBufferFlags);
+            d.a = __builtin_amdgcn_raw_buffer_load_b128(rp, 0, 0, 0);
+            d.b = __builtin_amdgcn_raw_buffer_load_b64(rp, 16, 0, 0);
+            d.c = __builtin_amdgcn_raw_buffer_load_b128(rq, 0, 0, 0);
+            d.d = __builtin_amdgcn_raw_buffer_load_b64(rq, 16, 0, 0);
+            d.e = __builtin_amdgcn_raw_buffer_load_b128(rq, 32, 0, 0);
+            d.f = __builtin_amdgcn_raw_buffer_load_b64(rq, 48, 0, 0);
+        };
+        // operands of one ray from its parameters (block 0 evaluated, blocks
+        // 1-3 rotated from it)
+        auto operands = [&](const RayRegs& d, float (&av)[4], float (&bv)[CB]) {
+            auto F = [](uint32_t u) { return __uint_as_float(u); };
+            const float f = F(d.b.y);
+            const float tha = step_phase(ci[0], F(d.a.x), F(d.a.y), F(d.b.x) + oa);
+            const float as = f * sin_rev(tha), ac = f * cos_rev(tha);
+            const float thb = step_phase(cj[0], F(d.a.z), F(d.a.w), ob);
+            const float bs = sin_rev(thb), bc = cos_rev(thb);
+            const float rc[3] = {F(d.c.x), F(d.c.z), F(d.d.x)};
+            const float rs[3] = {F(d.c.y), F(d.c.w), F(d.d.y)};
+            const float qc[3] = {F(d.e.x), F(d.e.z), F(d.f.x)};
+            const float qs[3] = {F(d.e.y), F(d.e.w), F(d.f.y)};
+            av[0] = as;
+            bv[0] = bs;
+            for (int k = 1; k < 4; k++) {
+                av[k] = __builtin_fmaf(as, rc[k - 1], ac * rs[k - 1]);
+                bv[k] = __builtin_fmaf(bs, qc[k - 1], bc * qs[k - 1]);
             }
         };
-        if (r0 < r1) fetch(cur, r0);
-        for (int r = r0; r < r1; r += G) {
-            if (r + G < r1) fetch(nxt, r + G);
-            for (int q = 0; q < G; q++) {
-                const float ah = cur[q][0], al = cur[q][1], bh = cur[q][2], bl = cur[q][3];
-                const float pa = cur[q][4] + oa, f = cur[q][5];
-                float av[4], bv[2];
-                for (int rb = 0; rb < 4; rb++)
-                    av[rb] = sin_rev(step_phase(ci[rb], ah, al, pa));
-                for (int cb = 0; cb < 2; cb++)
-                    bv[cb] = f * sin_rev(step_phase(cj[cb], bh, bl, ob));
-                for (int rb = 0; rb < 4; rb++)
-                    for (int cb = 0; cb < 2; cb++)
-                        acc[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[rb], bv[cb],
-                                                                          acc[rb][cb], 0, 0, 0);
+        // Software pipeline, one wave per SIMD.  Step r: the MFMAs of ray r
+        // run while the VALU computes the operands of ray r + 1, interleaved
+        // with them so that the VALU latencies hide under the MFMAs, and the
+        // loads of ray r + 3 are issued (two steps of MFMA time to arrive).
+        // Parameters rotate through three buffers and operands through two,
+        // so the loop is unrolled by six and needs no register copies; the
+        // scheduling barriers keep each stage in its step.  The step count
+        // is rounded up to a multiple of six; the extra steps see rays past
+        // r1, whose operands are zero.
+        RayRegs pb[3];
+        float av[2][4], bv[2][CB];
+        auto step = [&](int r, int ia, const RayRegs& use, RayRegs& issue) {
+            load(issue, r + 3);
+            __builtin_amdgcn_sched_barrier(0);
+            operands(use, av[ia ^ 1], bv[ia ^ 1]);
+            for (int rb = 0; rb < 4; rb++)
+                for (int cb = 0; cb < CB; cb++)
+                    acc[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[ia][rb], bv[ia][cb],
+                                                                      acc[rb][cb], 0, 0, 0);
+            for (int k = 0; k < 2 * CB; k++) {
+                __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);   // two MFMAs
+                __builtin_amdgcn_sched_group_barrier(0x002, 4, 0);   // four VALU
             }
-            for (int q = 0; q < G; q++)
-                for (int e = 0; e < 6; e++) cur[q][e] = nxt[q][e];
+            __builtin_amdgcn_sched_barrier(0);
+        };
+        if (r0 < r1) {
+            load(pb[0], r0);
+            load(pb[1], r0 + 1);
+            load(pb[2], r0 + 2);
+            operands(pb[0], av[0], bv[0]);
+            for (int r = r0; r < r1; r += 6) {
+                step(r, 0, pb[1], pb[0]);
+                step(r + 1, 1, pb[2], pb[1]);
+                step(r + 2, 0, pb[0], pb[2]);
+                step(r + 3, 1, pb[1], pb[0]);
+                step(r + 4, 0, pb[2], pb[1]);
+                step(r + 5, 1, pb[0], pb[2]);
+            }
         }
     } else {
         // 32x32x16 bf16: lane supplies A'[l32][8*half + e], e = 0..7, i.e. rays
         // 4*half + e/2, part e%2; same for B'.
-        for (int r = r0; r < r1; r += kRayGroup) {
-            v8bf ahi[4], alo[4], bhi[2], blo[2];
+        // Each (part 0, part 1) pair of a ray is one dword of packed bf16 in
+        // the operand, split as x = hi + lo by split_pair.  The phase
+        // parameters of a group's rays for this lane (r + 4 half + q) are
+        // buffer loads issued one group ahead of their use (two register sets
+        // used in turn); the row rotations are loaded at the top of the step
+        // and used after the column operands are made.  Rays at or past r1
+        // read zeros, so their flux is zero.
+        struct GroupRegs { v4u p[4]; v2u f[4]; };
+        auto rsrc = [&](const float* base, int r, int stride) {
+            // wave-uniform; read back as such (a clamp would otherwise be
+            // computed on the VALU and the resource taken as divergent)
+            const int nr = __builtin_amdgcn_readfirstlane(min(max(r1 - r, 0), kRayGroup));
+            return __builtin_amdgcn_make_buffer_rsrc(
+                (void*)(base + (int64_t)min(r, r1) * stride), (short)0, nr * stride * 4,
+                kBufferFlags);
+        };
+        auto gload = [&](GroupRegs& g, int r) {
+            const __amdgpu_buffer_rsrc_t rp = rsrc(params, r, kRayStride);
             for (int q = 0; q < 4; q++) {
-                const int rr = r + 4*half + q;
-                const bool live = rr < r1;
-                const float* p = params + (int64_t)(live ? rr : r0) * kRayStride;
-                const float ah = p[0], al = p[1], bh = p[2], bl = p[3], ph0 = p[4];
-                const float f = live ? p[5] : 0.f;
-                // row block 0 evaluated, blocks 1-3 rotated from it
-                const float th = step_phase(ci[0], ah, al, ph0);
-                const float fs0 = f * sin_rev(th), fc0 = f * cos_rev(th);
-                for (int rb = 0; rb < 4; rb++) {
-                    float s = fs0, c = fc0;
-                    if (rb > 0) {
-                        const float* pr = rot + (int64_t)(live ? rr : r0) * kRotStride;
-                        const float cr = pr[2*rb - 2], sr = pr[2*rb - 1];
-                        s = __builtin_fmaf(fs0, cr, fc0 * sr);
-                        c = __builtin_fmaf(fc0, cr, -(fs0 * sr));
-                    }
-                    const float v0 = prow ? s : c;        // k part 0
-                    const float v1 = prow ? c : -s;       // k part 1
-                    const __bf16 h0 = (__bf16)v0, h1 = (__bf16)v1;
-                    ahi[rb][2*q] = h0;
-                    ahi[rb][2*q+1] = h1;
-                    alo[rb][2*q] = (__bf16)(v0 - (float)h0);
-                    alo[rb][2*q+1] = (__bf16)(v1 - (float)h1);
-                }
-                for (int cb = 0; cb < 2; cb++) {
-                    const float th = step_phase(cj[cb], bh, bl, 0.f);
-                    const float c = cos_rev(th), s = sin_rev(th);
-                    const __bf16 hc = (__bf16)c, hs = (__bf16)s;
-                    bhi[cb][2*q] = hc;
-                    bhi[cb][2*q+1] = hs;
-                    blo[cb][2*q] = (__bf16)(c - (float)hc);
-                    blo[cb][2*q+1] = (__bf16)(s - (float)hs);
-                }
+                const int k = 4*half + q;
+                g.p[q] = __builtin_amdgcn_raw_buffer_load_b128(rp, k * kRayStride * 4, 0, 0);
+                g.f[q] = __builtin_amdgcn_raw_buffer_load_b64(rp, k * kRayStride * 4 + 16, 0, 0);
             }
-            for (int rb = 0; rb < 4; rb++)
+        };
+        auto op = [](const uint32_t (&w)[4]) {
+            return __builtin_bit_cast(v8bf, (v4u){w[0], w[1], w[2], w[3]});
+        };
+        auto F = [](uint32_t u) { return __uint_as_float(u); };
+        auto gstep = [&](int r, const GroupRegs& g, GroupRegs& next) {
+            gload(next, r + kRayGroup);
+            const __amdgpu_buffer_rsrc_t rq = rsrc(rot, r, kRotStride);
+            v4u rc[4];
+            v2u rd[4];
+            for (int q = 0; q < 4; q++) {
+                const int k = 4*half + q;
+                rc[q] = __builtin_amdgcn_raw_buffer_load_b128(rq, k * kRotStride * 4, 0, 0);
+                rd[q] = __builtin_amdgcn_raw_buffer_load_b64(rq, k * kRotStride * 4 + 16, 0, 0);
+            }
+            uint32_t bhi[2][4], blo[2][4];                    // [block][ray q]
+            for (int q = 0; q < 4; q++)
+                for (int cb = 0; cb < 2; cb++) {
+                    const float thb = step_phase(cj[cb], F(g.p[q].z), F(g.p[q].w), 0.f);
+                    split_pair(cos_rev(thb), sin_rev(thb), bhi[cb][q], blo[cb][q]);
+                }
+            // row block 0 evaluated, blocks 1-3 rotated from it; the lane's k
+            // parts (v0, v1) of U = c + i s are (c, -s) on a real row and
+            // (s, c) on an imaginary one, and both rotate as
+            // (x, y) -> (x cr + y sr, y cr - x sr)
+            float x[4], y[4];
+            for (int q = 0; q < 4; q++) {
+                const float th = step_phase(ci[0], F(g.p[q].x), F(g.p[q].y), F(g.f[q].x));
+                const float f = F(g.f[q].y);
+                const float fs0 = f * sin_rev(th), fc0 = f * cos_rev(th);
+                x[q] = prow ? fs0 : fc0;
+                y[q] = prow ? fc0 : -fs0;
+            }
+            for (int rb = 0; rb < 4; rb++) {
+                uint32_t ahi[4], alo[4];
+                for (int q = 0; q < 4; q++) {
+                    if (rb == 0) {
+                        split_pair(x[q], y[q], ahi[q], alo[q]);
+                    } else {
+                        const float cr = F(rb == 3 ? rd[q].x : rc[q][2*rb - 2]);
+                        const float sr = F(rb == 3 ? rd[q].y : rc[q][2*rb - 1]);
+                        split_pair(__builtin_fmaf(x[q], cr, y[q] * sr),
+                                   __builtin_fmaf(y[q], cr, -(x[q] * sr)), ahi[q], alo[q]);
+                    }
+                }
                 for (int cb = 0; cb < 2; cb++) {
                     v16f a = acc[rb][cb];
-                    a = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo[rb], bhi[cb], a, 0, 0, 0);
-                    a = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi[rb], blo[cb], a, 0, 0, 0);
-                    a = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi[rb], bhi[cb], a, 0, 0, 0);
+                    a = __builtin_amdgcn_mfma_f32_32x32x16_bf16(op(alo), op(bhi[cb]), a, 0, 0, 0);
+                    a = __builtin_amdgcn_mfma_f32_32x32x16_bf16(op(ahi), op(blo[cb]), a, 0, 0, 0);
+                    a = __builtin_amdgcn_mfma_f32_32x32x16_bf16(op(ahi), op(bhi[cb]), a, 0, 0, 0);
                     acc[rb][cb] = a;
                 }
+            }
+        };
+        // unrolled by two so the register sets alternate without copies; a
+        // step past r1 reads zeros and adds nothing
+        if (r0 < r1) {
+            GroupRegs g0, g1;
+            gload(g0, r0);
+            for (int r = r0; r < r1; r += 2 * kRayGroup) {
+                gstep(r, g0, g1);
+                gstep(r + kRayGroup, g1, g0);
+            }
         }
     }
 
     // D[row][col]: row = 8*(e/4) + 4*half + e%4, col = l32 (32x32 MFMA layout)
     // out[i][j][2] over the padded nxp x nyp image
+#pragma unroll
     for (int rb = 0; rb < 4; rb++)
-        for (int cb = 0; cb < 2; cb++)
+#pragma unroll
+        for (int cb = 0; cb < CB; cb++)
+#pragma unroll
             for (int e = 0; e < 16; e++) {
                 const int row = 8*(e >> 2) + 4*half + (e & 3);
                 const int i = i0 + rb*16 + (row >> 1);
@@ -319,7 +460,7 @@ __device__ __forceinline__ void psf_tile(const float* __restrict__ params,
 
 
 template <int kPrec>
-__global__ __launch_bounds__(256, 2) void k_psf_gemm(
+__global__ __launch_bounds__(gemm_threads<kPrec>(), gemm_waves_per_simd<kPrec>()) void k_psf_gemm(
     const float* __restrict__ params, const float* __restrict__ rot,
     const int* __restrict__ total_ptr, int nx, int ny, int tiles_i, int ksplit, int nxp,
     int nyp, float* __restrict__ partial)
@@ -556,7 +697,7 @@ __global__ __launch_bounds__(kScanThreads) void k_psf_items(SegRange* __restrict
 // Persistent GEMM: workgroup w takes (item, tile) pairs w, w + grid, ... in a
 // fixed assignment, so the result does not depend on scheduling.
 template <int kPrec>
-__global__ __launch_bounds__(256, 2) void k_psf_gemm_batch(
+__global__ __launch_bounds__(gemm_threads<kPrec>(), gemm_waves_per_simd<kPrec>()) void k_psf_gemm_batch(
     const float* __restrict__ params, const float* __restrict__ rot,
     const WorkItem* __restrict__ items,
     const int* __restrict__ nitems_ptr, int tiles_i, int tiles, int nx, int ny, int nxp,
@@ -699,10 +840,10 @@ extern "C" int bt_huygens_gemm(bt_rays rays, const double* p0, const double* a1,
     const int* total = offsets + (rays.n > 0 ? nb : 0);
     dim3 grid(L.tiles_i * L.tiles_j, L.ksplit);
     if (precision == BT_PSF_FP32)
-        hipLaunchKernelGGL(k_psf_gemm<BT_PSF_FP32>, grid, dim3(256), 0, s, params, rot, total, nx, ny,
+        hipLaunchKernelGGL(k_psf_gemm<BT_PSF_FP32>, grid, dim3(gemm_threads<BT_PSF_FP32>()), 0, s, params, rot, total, nx, ny,
                            L.tiles_i, L.ksplit, L.nxp, L.nyp, partial);
     else
-        hipLaunchKernelGGL(k_psf_gemm<BT_PSF_BF16X3>, grid, dim3(256), 0, s, params, rot, total, nx, ny,
+        hipLaunchKernelGGL(k_psf_gemm<BT_PSF_BF16X3>, grid, dim3(gemm_threads<BT_PSF_BF16X3>()), 0, s, params, rot, total, nx, ny,
                            L.tiles_i, L.ksplit, L.nxp, L.nyp, partial);
     rc = check_launch("bt_huygens_gemm(mfma)");
     if (rc) return rc;
@@ -825,10 +966,12 @@ extern "C" int bt_huygens_batch(bt_rays rays, const bt_psf_segment* segments, in
     const int grid = nwg > 0 ? nwg : 512;
     const int tiles = L.tiles_i * L.tiles_j;
     if (precision == BT_PSF_FP32)
-        hipLaunchKernelGGL(k_psf_gemm_batch<BT_PSF_FP32>, dim3(grid), dim3(256), 0, s, params,
+        hipLaunchKernelGGL(k_psf_gemm_batch<BT_PSF_FP32>, dim3(grid),
+                           dim3(gemm_threads<BT_PSF_FP32>()), 0, s, params,
                            rot, items, nitems, L.tiles_i, tiles, nx, ny, L.nxp, L.nyp, partial);
     else
-        hipLaunchKernelGGL(k_psf_gemm_batch<BT_PSF_BF16X3>, dim3(grid), dim3(256), 0, s, params,
+        hipLaunchKernelGGL(k_psf_gemm_batch<BT_PSF_BF16X3>, dim3(grid),
+                           dim3(gemm_threads<BT_PSF_BF16X3>()), 0, s, params,
                            rot, items, nitems, L.tiles_i, tiles, nx, ny, L.nxp, L.nyp, partial);
     rc = check_launch("bt_huygens_batch(mfma)");
     if (rc) return rc;
