@@ -121,7 +121,8 @@ __global__ __launch_bounds__(kScanThreads) void k_psf_scan(const int* __restrict
 }
 
 // Per ray: phase at p0 (revolutions) and per-pixel phase steps
-//   o   = [alpha_hi, alpha_lo, beta_hi, beta_lo, phi0_rev, flux, 0, 0]
+//   o   = [alpha_hi, alpha_lo, beta_hi, beta_lo, phi0_rev, flux,
+//          cos, sin of 2 pi frac(32 beta)]
 // and the rotations between the row blocks of a wave's tile (16, 32, 48
 // pixels apart) and between its column blocks (32, 64, 96 pixels apart),
 // with which blocks 1-3 are derived from block 0
@@ -157,8 +158,6 @@ __device__ __forceinline__ void prep_ray(const bt_rays& rv, int64_t i, double p0
     o[3] = (float)(be - (double)bh);
     o[4] = (float)rev;
     o[5] = (float)rv.flux[i];
-    o[6] = 0.f;
-    o[7] = 0.f;
     // one FP64 sincos per axis; the 2- and 3-block rotations by products
     // (relative error ~1e-16, far below the FP32 rounding of the table)
     auto blocks = [](double step, float* o) {
@@ -177,6 +176,8 @@ __device__ __forceinline__ void prep_ray(const bt_rays& rv, int64_t i, double p0
     rot[7] = 0.f;
     rot[14] = 0.f;
     rot[15] = 0.f;
+    o[6] = rot[8];                  // the one column rotation split-BF16 uses
+    o[7] = rot[9];
 }
 
 // The parameters of every valid ray, compacted in ray order.
@@ -358,10 +359,10 @@ __device__ __forceinline__ void psf_tile(const float* __restrict__ params,
         // the operand, split as x = hi + lo by split_pair.  The phase
         // parameters of a group's rays for this lane (r + 4 half + q) are
         // buffer loads issued one group ahead of their use (two register sets
-        // used in turn); the row rotations are loaded at the top of the step
-        // and used after the column operands are made.  Rays at or past r1
+        // used in turn); the block rotations are loaded at the top of the
+        // step and used after the phases are evaluated.  Rays at or past r1
         // read zeros, so their flux is zero.
-        struct GroupRegs { v4u p[4]; v2u f[4]; };
+        struct GroupRegs { v4u p[4]; v4u f[4]; };
         auto rsrc = [&](const float* base, int r, int stride) {
             // wave-uniform; read back as such (a clamp would otherwise be
             // computed on the VALU and the resource taken as divergent)
@@ -375,7 +376,7 @@ __device__ __forceinline__ void psf_tile(const float* __restrict__ params,
             for (int q = 0; q < 4; q++) {
                 const int k = 4*half + q;
                 g.p[q] = __builtin_amdgcn_raw_buffer_load_b128(rp, k * kRayStride * 4, 0, 0);
-                g.f[q] = __builtin_amdgcn_raw_buffer_load_b64(rp, k * kRayStride * 4 + 16, 0, 0);
+                g.f[q] = __builtin_amdgcn_raw_buffer_load_b128(rp, k * kRayStride * 4 + 16, 0, 0);
             }
         };
         auto op = [](const uint32_t (&w)[4]) {
@@ -392,12 +393,16 @@ __device__ __forceinline__ void psf_tile(const float* __restrict__ params,
                 rc[q] = __builtin_amdgcn_raw_buffer_load_b128(rq, k * kRotStride * 4, 0, 0);
                 rd[q] = __builtin_amdgcn_raw_buffer_load_b64(rq, k * kRotStride * 4 + 16, 0, 0);
             }
+            // column block 0 evaluated, block 1 (32 pixels on) rotated from it
             uint32_t bhi[2][4], blo[2][4];                    // [block][ray q]
-            for (int q = 0; q < 4; q++)
-                for (int cb = 0; cb < 2; cb++) {
-                    const float thb = step_phase(cj[cb], F(g.p[q].z), F(g.p[q].w), 0.f);
-                    split_pair(cos_rev(thb), sin_rev(thb), bhi[cb][q], blo[cb][q]);
-                }
+            for (int q = 0; q < 4; q++) {
+                const float thb = step_phase(cj[0], F(g.p[q].z), F(g.p[q].w), 0.f);
+                const float c = cos_rev(thb), sn = sin_rev(thb);
+                split_pair(c, sn, bhi[0][q], blo[0][q]);
+                const float cq = F(g.f[q].z), sq = F(g.f[q].w);
+                split_pair(__builtin_fmaf(c, cq, -(sn * sq)), __builtin_fmaf(sn, cq, c * sq),
+                           bhi[1][q], blo[1][q]);
+            }
             // row block 0 evaluated, blocks 1-3 rotated from it; the lane's k
             // parts (v0, v1) of U = c + i s are (c, -s) on a real row and
             // (s, c) on an imaginary one, and both rotate as
