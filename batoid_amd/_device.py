@@ -8,18 +8,31 @@ import torch
 from . import _lib
 
 
+_ready = False
+
+
 def device():
-    if not torch.cuda.is_available():
-        raise _lib.BatoidAMDError(
-            "batoid_amd needs a ROCm GPU (torch.cuda.is_available() is False)"
-        )
-    _lib.load()
+    global _ready
+    if not _ready:
+        if not torch.cuda.is_available():
+            raise _lib.BatoidAMDError(
+                "batoid_amd needs a ROCm GPU (torch.cuda.is_available() is False)"
+            )
+        _lib.load()
+        _ready = True
     return torch.device("cuda", torch.cuda.current_device())
+
+
+# torch's raw current-stream query (what its own generated code calls) when it
+# has one; the public Stream object otherwise
+_raw_stream = getattr(torch._C, "_cuda_getCurrentRawStream", None)
 
 
 def stream():
     """Handle of torch's current HIP stream: library calls are ordered with
     torch's own work on it."""
+    if _raw_stream is not None:
+        return ctypes.c_void_p(_raw_stream(torch.cuda.current_device()))
     return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 

@@ -157,28 +157,46 @@ __global__ __launch_bounds__(kBlock) void k_select_count(bt_rays in, double min_
     if (threadIdx.x == 0) counts[blockIdx.x] = total;
 }
 
-// Exclusive scan of the block counts (one workgroup, chunks of 256 in order).
+// Exclusive scan of the block counts by one workgroup, in chunks of
+// kBlock * kScanPer counts taken in order: each thread sums kScanPer
+// consecutive counts, the thread sums are scanned within each wave by
+// shuffles and across the four waves through LDS (two barriers per chunk).
+constexpr int kScanPer = 4;
+
 __global__ __launch_bounds__(kBlock) void k_select_scan(const int64_t* __restrict__ counts,
                                                         int64_t nblocks,
                                                         int64_t* __restrict__ offsets,
                                                         int64_t* __restrict__ total)
 {
-    __shared__ int64_t buf[kBlock];
+    __shared__ int64_t wave_sum[kBlock / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     int64_t carry = 0;
-    for (int64_t base = 0; base < nblocks; base += kBlock) {
-        const int64_t b = base + threadIdx.x;
-        const int64_t c = b < nblocks ? counts[b] : 0;
-        buf[threadIdx.x] = c;
-        __syncthreads();
-        for (int d = 1; d < kBlock; d <<= 1) {       // inclusive Hillis-Steele scan
-            const int64_t v = threadIdx.x >= d ? buf[threadIdx.x - d] : 0;
-            __syncthreads();
-            buf[threadIdx.x] += v;
-            __syncthreads();
+    for (int64_t base = 0; base < nblocks; base += (int64_t)kBlock * kScanPer) {
+        const int64_t b0 = base + (int64_t)threadIdx.x * kScanPer;
+        int64_t c[kScanPer];
+        int64_t sum = 0;
+        for (int k = 0; k < kScanPer; k++) {
+            c[k] = b0 + k < nblocks ? counts[b0 + k] : 0;
+            sum += c[k];
         }
-        if (b < nblocks) offsets[b] = carry + buf[threadIdx.x] - c;
-        const int64_t chunk = buf[kBlock - 1];
+        int64_t incl = sum;                          // inclusive scan within the wave
+        for (int d = 1; d < 64; d <<= 1) {
+            const int64_t v = __shfl_up(incl, d, 64);
+            if (lane >= d) incl += v;
+        }
+        if (lane == 63) wave_sum[wave] = incl;
         __syncthreads();
+        int64_t before = carry, chunk = 0;
+        for (int w = 0; w < kBlock / 64; w++) {
+            if (w < wave) before += wave_sum[w];
+            chunk += wave_sum[w];
+        }
+        __syncthreads();                             // wave_sum is rewritten next chunk
+        int64_t run = before + incl - sum;
+        for (int k = 0; k < kScanPer; k++) {
+            if (b0 + k < nblocks) offsets[b0 + k] = run;
+            run += c[k];
+        }
         carry += chunk;
     }
     if (threadIdx.x == 0) *total = carry;

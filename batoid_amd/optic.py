@@ -12,6 +12,7 @@ from . import _lib, _device
 from .constants import globalCoordSys, vacuum
 from .obscuration import ObscNegation, ObscCircle, ObscAnnulus
 from .coating import SimpleCoating
+from .rayVector import RayVector
 
 
 _DERIVED = ('path', 'itemDict', '_names', 'R_inner', 'R_outer', 'inRadius',
@@ -504,18 +505,18 @@ class CompoundOptic(Optic):
             item = self.items[index]
             rF, rR = item.traceSplit(rays, minFlux=minFlux, reverse=(direction == "reverse"),
                                      _verbose=_verbose)
-            lists = []
-            for rl in (rF, rR):
-                kept = []
+            # the surviving rays of every output, filtered together (one
+            # host synchronisation for all their counts)
+            outs = []
+            for side, rl in enumerate((rF, rR)):
                 if not isinstance(rl, list):           # a skipped item passes rays on
                     rl = [rl]
-                for rr in rl:
-                    if not rr:
-                        continue
-                    sel = rr._select(minFlux)
-                    sel.path = rr.path
-                    kept.append(sel)
-                lists.append(kept)
+                outs += [(side, rr) for rr in rl if rr]
+            sels = RayVector._select_many([rr for _, rr in outs], minFlux)
+            lists = [[], []]
+            for (side, rr), sel in zip(outs, sels):
+                sel.path = rr.path
+                lists[side].append(sel)
             for rr in lists[0]:
                 if len(rr) > 0:
                     if index == len(self.items) - 1:

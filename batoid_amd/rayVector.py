@@ -44,24 +44,35 @@ class RayVector:
                 val = val.detach().cpu().numpy()
             cols[name] = _device.u8(np.broadcast_to(np.asarray(val, bool), shape).ravel())
         self._cols = cols
+        self._buf = None
         self.coordSys = coordSys
 
     @staticmethod
-    def _directInit(cols, coordSys):
-        """Wrap existing device columns (no copy)."""
+    def _directInit(cols, coordSys, buf=None):
+        """Wrap existing device columns (no copy); ``buf`` is the one buffer
+        they are views of, when they are."""
         ret = RayVector.__new__(RayVector)
         ret._cols = cols
+        ret._buf = buf
         ret.coordSys = coordSys
         return ret
 
     @staticmethod
     def _empty(n, coordSys):
-        """n rays of uninitialised device columns."""
-        dev = _device.device()
-        cols = {k: torch.empty(int(n), dtype=torch.float64, device=dev) for k in _FCOLS}
-        for k in _BCOLS:
-            cols[k] = torch.empty(int(n), dtype=torch.uint8, device=dev)
-        return RayVector._directInit(cols, coordSys)
+        """n rays of uninitialised device columns, views of one allocation
+        (each column starting on a 256-byte boundary)."""
+        n = int(n)
+        sf = -(-n // 32) * 32                  # float64 column stride, elements
+        sb = -(-n // 256) * 256                # uint8 column stride, bytes
+        nf = len(_FCOLS)
+        buf = torch.empty(nf * sf * 8 + len(_BCOLS) * sb, dtype=torch.uint8,
+                          device=_device.device())
+        f = buf[: nf * sf * 8].view(torch.float64)
+        cols = {k: f[i * sf: i * sf + n] for i, k in enumerate(_FCOLS)}
+        base = nf * sf * 8
+        for j, k in enumerate(_BCOLS):
+            cols[k] = buf[base + j * sb: base + j * sb + n]
+        return RayVector._directInit(cols, coordSys, buf)
 
     def _select(self, minFlux):
         """The rays that are not vignetted and have flux >= minFlux, in order,
@@ -69,18 +80,46 @@ class RayVector:
         library's two-pass compaction."""
         lib = _lib.load()
         n = len(self)
-        ws = torch.empty(int(lib.bt_select_workspace_bytes(n)), dtype=torch.uint8,
-                         device=_device.device())
-        count = torch.zeros(1, dtype=torch.int64, device=ws.device)
+        nws = int(lib.bt_select_workspace_bytes(n))
+        ws = torch.empty(nws + 8, dtype=torch.uint8, device=_device.device())
+        count = ws[nws:].view(torch.int64)             # written by bt_select_count
         stream = _device.stream()
         _lib.call("bt_select_count", self._rays, float(minFlux), _device.ptr(count),
-                  _device.ptr(ws), ws.numel(), stream)
+                  _device.ptr(ws), nws, stream)
         m = int(count.item())
         out = RayVector._empty(m, self.coordSys)
         if m:
             _lib.call("bt_select_scatter", self._rays, float(minFlux), out._rays,
-                      _device.ptr(ws), ws.numel(), stream)
+                      _device.ptr(ws), nws, stream)
         return out
+
+    @staticmethod
+    def _select_many(rvs, minFlux):
+        """``[rv._select(minFlux) for rv in rvs]`` with one host
+        synchronisation for all the counts instead of one per RayVector."""
+        if not rvs:
+            return []
+        lib = _lib.load()
+        stream = _device.stream()
+        sizes = [int(lib.bt_select_workspace_bytes(len(rv))) for rv in rvs]
+        offs, tot = [], 0
+        for b in sizes:
+            offs.append(tot)
+            tot += -(-b // 256) * 256
+        ws = torch.empty(tot + 8 * len(rvs), dtype=torch.uint8, device=_device.device())
+        counts = ws[tot:].view(torch.int64)            # written by bt_select_count
+        for i, rv in enumerate(rvs):
+            _lib.call("bt_select_count", rv._rays, float(minFlux), _device.ptr(counts[i:]),
+                      _device.ptr(ws[offs[i]:]), sizes[i], stream)
+        ms = counts.tolist()
+        outs = []
+        for i, rv in enumerate(rvs):
+            out = RayVector._empty(ms[i], rv.coordSys)
+            if ms[i]:
+                _lib.call("bt_select_scatter", rv._rays, float(minFlux), out._rays,
+                          _device.ptr(ws[offs[i]:]), sizes[i], stream)
+            outs.append(out)
+        return outs
 
     @classmethod
     def fromTorch(cls, x, y, z, vx, vy, vz, t, wavelength, flux, vignetted, failed,
@@ -202,8 +241,13 @@ class RayVector:
         return self
 
     def copy(self):
-        return RayVector._directInit({k: v.clone() for k, v in self._cols.items()},
-                                     self.coordSys.copy())
+        out = RayVector._empty(len(self), self.coordSys.copy())
+        if self._buf is not None and self._buf.numel() == out._buf.numel():
+            out._buf.copy_(self._buf)                  # same layout: one copy
+        else:
+            for k, v in self._cols.items():
+                out._cols[k].copy_(v)
+        return out
 
     def __getitem__(self, idx):
         n = len(self)
@@ -242,6 +286,7 @@ class RayVector:
     def __setstate__(self, args):
         self._cols = {k: (_device.u8(a) if k in _BCOLS else _device.f64(a))
                       for k, a in zip(_COLS, args[:-1])}
+        self._buf = None
         self.coordSys = args[-1]
 
     # ---- generators ----------------------------------------------------------
