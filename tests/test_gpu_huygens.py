@@ -68,6 +68,26 @@ def test_gemm_matches_fp64(precision, nx, ny):
     assert err < 1e-5
 
 
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3"])
+@pytest.mark.parametrize("nkeep", [1, 5, 13, 700])
+def test_gemm_few_rays(precision, nkeep):
+    """K splits of fewer rays than the kernels' pipeline depth or ray group
+    (and many empty splits): the rays past a split's end must add nothing."""
+    print(f"case few rays {precision} {nkeep}", flush=True)
+    optic, rays = _traced(nx=64)
+    v = rays.torch("vignetted")
+    live = torch.nonzero(v == 0).flatten()
+    v[live[nkeep:]] = 1
+    point = rays.r[live[0].item()]
+    prim = np.array([[0.5e-6, 0.0], [0.0, 0.5e-6]])
+    time = float(rays.t[0])
+    for nx, ny in [(64, 64), (200, 136)]:
+        ref = huygens_amplitudes(rays, point, prim, nx, ny, time, method='direct')
+        got = huygens_amplitudes(rays, point, prim, nx, ny, time, method='gemm',
+                                 precision=precision)
+        assert rel_l2(got, ref) < 1e-5
+
+
 def test_gemm_sheared_lattice_and_all_rays():
     """Non-diagonal primitive vectors, vignetted rays included."""
     optic, rays = _traced(nx=64, theta=0.3)
