@@ -52,7 +52,8 @@ def _reference_point(rays, nx, reference):
         w = np.where(1 - rays.vignetted)[0]
         return np.mean(rays.r[w], axis=0)
     if reference == 'chief':
-        return rays.r[_chief_index(nx)]
+        k = _chief_index(nx)             # one ray's position read back, not the grid's
+        return torch.stack([rays.torch(c)[k] for c in ("x", "y", "z")]).cpu().numpy()
     raise ValueError(f"unknown reference {reference!r}")
 
 
@@ -169,6 +170,9 @@ def zernike_table(jmax, eps=0.0):
     return C, nmax
 
 
+_COEF_CACHE = {}     # (jmax, eps, device) -> (monomial table on the device, order)
+
+
 def zernike_gram(x, y, t, t0, wavelength, R_outer, R_inner=0.0, jmax=22, flux=None,
                  mask=None):
     """Gram matrix of [Z_1 .. Z_jmax, w] over the rays not masked, with
@@ -177,10 +181,14 @@ def zernike_gram(x, y, t, t0, wavelength, R_outer, R_inner=0.0, jmax=22, flux=No
     columns.  Returns the (jmax+1, jmax+1) symmetric matrix (host)."""
     if not 1 <= jmax <= 78:
         raise ValueError("jmax must be in [1, 78]")
-    C, nmax = zernike_table(jmax, R_inner / R_outer if R_inner else 0.0)
+    eps = R_inner / R_outer if R_inner else 0.0
     lib = _lib.load()
     dev = _device.device()
-    coef = _device.f64(C)
+    key = (jmax, eps, dev.index)
+    if key not in _COEF_CACHE:
+        C, nmax = zernike_table(jmax, eps)
+        _COEF_CACHE[key] = (_device.f64(C), nmax)
+    coef, nmax = _COEF_CACHE[key]
     ws = torch.empty(int(lib.bt_zernike_workspace_bytes(jmax)), dtype=torch.uint8, device=dev)
     nent = (jmax + 1) * (jmax + 2) // 2
     out = torch.empty(nent, dtype=torch.float64, device=dev)
@@ -219,7 +227,7 @@ def zernike(optic, theta_x, theta_y, wavelength, projection='postel', nx=32,
     G = zernike_gram(epRays.torch("x"), epRays.torch("y"), srays.torch("t"), t0, wavelength,
                      optic.pupilSize / 2, optic.pupilSize / 2 * eps, jmax, mask=mask)
     A, b = G[:jmax, :jmax], G[:jmax, jmax]
-    nrows = int((mask == 0).sum().item())
+    nrows = int(round(G[0, 0]))            # Z_1 = 1: the count of unmasked rays
     s = np.sqrt(np.clip(np.linalg.eigvalsh(A), 0.0, None))
     rank = int((s > s.max() * max(nrows, jmax + 1) * np.finfo(float).eps).sum())
     if rank < jmax:

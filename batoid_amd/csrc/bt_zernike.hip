@@ -67,9 +67,11 @@ __global__ __launch_bounds__(kZThreads) void k_zernike_gram(ZArgs a)
     }
     for (int64_t base = (int64_t)blockIdx.x * kZChunk; base < a.n;
          base += (int64_t)gridDim.x * kZChunk) {
+        const int64_t i = base + tid;
+        const bool ok = tid < kZChunk && i < a.n && !(a.mask && a.mask[i]);
+        // a chunk whose rays are all masked adds nothing (outside the pupil)
+        if (!__syncthreads_or(ok)) continue;
         if (tid < kZChunk) {
-            const int64_t i = base + tid;
-            const bool ok = i < a.n && !(a.mask && a.mask[i]);
             double u = 0.0, v = 0.0, w = 0.0;
             if (ok) {
                 u = a.x[i] * a.inv_r;
@@ -140,15 +142,30 @@ __global__ __launch_bounds__(kZThreads) void k_zernike_gram(ZArgs a)
     }
 }
 
+// out[e] = sum over workgroups b of partial[b][e], in a fixed order: a
+// block covers kZRedEnt consecutive entries (coalesced reads) with
+// kZRedSlices lanes per entry, lane g summing b = g, g + kZRedSlices, ...;
+// the slice sums are then added in g order.
+constexpr int kZRedEnt = 32;
+constexpr int kZRedSlices = 256 / kZRedEnt;
+
 __global__ __launch_bounds__(256) void k_zernike_reduce(const double* __restrict__ partial,
                                                         int nblocks, int nent,
                                                         double* __restrict__ out)
 {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= nent) return;
+    __shared__ double part[kZRedSlices][kZRedEnt];
+    const int pe = threadIdx.x % kZRedEnt, g = threadIdx.x / kZRedEnt;
+    const int e = blockIdx.x * kZRedEnt + pe;
     double s = 0.0;
-    for (int b = 0; b < nblocks; b++) s += partial[(int64_t)b * nent + e];
-    out[e] = s;
+    if (e < nent)
+        for (int b = g; b < nblocks; b += kZRedSlices) s += partial[(int64_t)b * nent + e];
+    part[g][pe] = s;
+    __syncthreads();
+    if (g == 0 && e < nent) {
+        double t = 0.0;
+        for (int k = 0; k < kZRedSlices; k++) t += part[k][pe];
+        out[e] = t;
+    }
 }
 
 }  // namespace bt
@@ -192,8 +209,8 @@ int bt_zernike_gram(const double* x, const double* y, const double* t, const dou
     hipLaunchKernelGGL(k_zernike_gram, dim3(kZGrid), dim3(kZThreads), 0, s, a);
     int rc = check_launch("bt_zernike_gram");
     if (rc) return rc;
-    hipLaunchKernelGGL(k_zernike_reduce, dim3((a.nent + 255) / 256), dim3(256), 0, s,
-                       (const double*)a.partial, kZGrid, a.nent, gram);
+    hipLaunchKernelGGL(k_zernike_reduce, dim3((a.nent + kZRedEnt - 1) / kZRedEnt), dim3(256), 0,
+                       s, (const double*)a.partial, kZGrid, a.nent, gram);
     return check_launch("bt_zernike_gram");
 }
 
