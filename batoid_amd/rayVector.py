@@ -24,6 +24,20 @@ _BCOLS = ("vignetted", "failed")
 _COLS = _FCOLS + _BCOLS
 
 
+def _hexapolar_rings(outer, inner, nrad, naz, kfold=6):
+    """Radii, azimuth counts and azimuth steps of the hexapolar rings,
+    outermost first (utils.py:512-531): nphi = (naz rho / outer) // kfold *
+    kfold (kfold when that is 0; 1 at rho = 0 when inner is 0), with the
+    same float64 operations as the reference's per-ring loop, elementwise."""
+    rhos = np.linspace(outer, inner, nrad)
+    nphis = (np.floor_divide(naz * rhos / outer, kfold)).astype(np.int64) * kfold
+    nphis[nphis == 0] = kfold
+    if inner == 0.0:
+        nphis[-1] = 1
+        rhos[-1] = 0.0
+    return rhos, nphis, 2 * np.pi / nphis
+
+
 class RayVector:
     """Rays as device columns.  Arguments broadcast against each other and
     are copied (rayVector.py:21-77)."""
@@ -496,17 +510,9 @@ class RayVector:
         POLAR generator: ray j of ring k sits at th = j * (2 pi / nphi_k),
         numpy's linspace(0, 2 pi, nphi_k, endpoint=False).  Returns the ray
         count and the device tables (keep them alive until the launch)."""
-        rhos = np.linspace(outer, inner, nrad)
-        nphis = []
-        for rho in rhos:
-            nphi = int((naz * rho / outer) // kfold) * kfold
-            nphis.append(nphi if nphi != 0 else kfold)
-        if inner == 0.0:
-            nphis[-1] = 1
-            rhos[-1] = 0.0
+        rhos, nphis, dth = _hexapolar_rings(outer, inner, nrad, naz, kfold)
         off = np.zeros(len(nphis) + 1, dtype=np.int64)
         off[1:] = np.cumsum(nphis)
-        dth = np.array([2 * np.pi / k for k in nphis])
         off_d = torch.from_numpy(off).to(_device.device())
         rho_d, dth_d = _device.f64(rhos), _device.f64(dth)
         gen.pattern = _lib.GEN_POLAR

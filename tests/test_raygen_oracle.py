@@ -213,3 +213,24 @@ def test_hypot_restatement_is_libm():
                 and 1e-150 < abs(y) < 1e150:
             cr_differs += math.sqrt(x * x + y * y) != a
     assert cr_differs > 50
+
+
+def test_hexapolar_rings_vectorised():
+    """The device ring table's radii, counts and azimuth steps equal the
+    reference's per-ring loop (the oracle's hexapolar), element for element."""
+    from batoid_amd.rayVector import _hexapolar_rings
+    rng = np.random.default_rng(11)
+    cases = [(4.18, 2.558, 1120, 11080, 6), (1.0, 0.0, 7, 30, 6), (0.5, 0.1, 3, 5, 6),
+             (2.0, 0.0, 50, 301, 4)]
+    cases += [(rng.uniform(0.1, 5), rng.uniform(0, 0.09), int(rng.integers(1, 200)),
+               int(rng.integers(1, 5000)), int(rng.integers(1, 9))) for _ in range(40)]
+    for outer, inner, nrad, naz, kfold in cases:
+        rhos, nphis, dth = _hexapolar_rings(outer, inner, nrad, naz, kfold)
+        rr, th = G.hexapolar(outer, inner, nrad, naz, kfold)
+        assert int(nphis.sum()) == len(rr)
+        off = np.concatenate([[0], np.cumsum(nphis)])
+        for k in range(len(nphis)):
+            np.testing.assert_array_equal(rr[off[k]:off[k + 1]], rhos[k])
+            assert dth[k] == 2 * np.pi / int(nphis[k])
+            if nphis[k] > 1:
+                assert th[off[k] + 1] == dth[k]
