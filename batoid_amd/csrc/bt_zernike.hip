@@ -12,12 +12,21 @@
 // which holds the least-squares normal matrix (r, c < J), the projections
 // sum_i Z_j w (c = J) and sum_i w^2.  The host solves the J x J system.
 //
-// Each workgroup takes chunks of 64 rays: monomials and wavefront values to
-// LDS, then the basis values (a 64-ray x J product with C, read through
-// wave-uniform scalar loads), then every thread adds its share of the
-// Gram entries over the chunk.  Per-workgroup partial Gram matrices are
-// summed in workgroup order by a second kernel, so results do not depend on
-// scheduling.
+// Each workgroup stages the coefficient table in LDS and takes chunks of 64
+// rays; both products run on the FP64
+// matrix cores (v_mfma_f64_16x16x4f64: lane l supplies A[l%16][l/16] and
+// B[l/16][l%16] and holds D[4e + l/16][l%16], e = 0..3):
+//  1. monomials of the 64 rays and their wavefront values to LDS;
+//  2. the basis values Z[p][j] = sum_m mono[p][m] C[j][m]: wave w takes rays
+//     16w .. 16w+15 against every 16-column block of C (K = monomials);
+//  3. the Gram blocks G[16 rb..][16 cb..] (rb <= cb) += sum_p B[p][r] B[p][c]
+//     over the same 16 rays (K = rays), accumulated in registers.
+// A wave only reads the basis columns it wrote, so steps 2-3 need no
+// barrier.  At the end the four waves' Gram blocks are added in wave order
+// in LDS and the workgroup writes its partial Gram matrix (upper triangle,
+// row-major); a second kernel sums the partials in workgroup order, so the
+// result does not depend on scheduling.  Chunks whose rays are all masked
+// are skipped (they add exact zeros).
 #include <hip/hip_runtime.h>
 #include <string>
 
@@ -25,14 +34,14 @@
 
 namespace bt {
 
+typedef double v4d __attribute__((ext_vector_type(4)));
+
 constexpr int kZChunk = 64;                                // rays per LDS chunk
 constexpr int kZMaxN = 11;                                 // radial order
 constexpr int kZMaxM = (kZMaxN + 1) * (kZMaxN + 2) / 2;    // 78 monomials
 constexpr int kZMaxJ = 78;                                 // basis functions
-constexpr int kZMaxEnt = (kZMaxJ + 1) * (kZMaxJ + 2) / 2;  // 3160 Gram entries
 constexpr int kZThreads = 256;
-constexpr int kZPer = (kZMaxEnt + kZThreads - 1) / kZThreads;   // 13 per thread
-constexpr int kZGrid = 512;   // two 81 KB-LDS workgroups per CU
+constexpr int kZGrid = 512;
 
 struct ZArgs {
     const double* x;
@@ -47,37 +56,76 @@ struct ZArgs {
     double* partial;          // [gridDim.x][nent]
 };
 
+// K steps (4 monomials each) of the basis product for NB = ceil((J + 1) / 16):
+// the largest J with that NB has radial order 4, 7, 9, 10, 11, i.e. 15, 36,
+// 55, 66, 78 monomials
+constexpr int zernike_ksteps(int nb)
+{
+    return nb == 1 ? 4 : nb == 2 ? 9 : nb == 3 ? 14 : nb == 4 ? 17 : 20;
+}
+
+// NB = ceil((J + 1) / 16) Gram blocks per side
+template <int NB>
 __global__ __launch_bounds__(kZThreads) void k_zernike_gram(ZArgs a)
 {
-    __shared__ double mono[kZMaxM][kZChunk];
-    __shared__ double basis[kZMaxJ + 1][kZChunk + 1];   // +1: rows on distinct banks
+    constexpr int NBLK = NB * (NB + 1) / 2;
+    constexpr int KS = zernike_ksteps(NB);
+    constexpr int MP = KS * 4, JP = NB * 16;         // padded monomial and basis counts
+    __shared__ double mono[MP][kZChunk];    // [monomial][ray]; later the wave sums
+    __shared__ double basis[kZChunk][JP];   // [ray][basis j]; j = J is w
+    __shared__ double ct[MP][JP];           // C transposed, [monomial][basis j]
     const int tid = threadIdx.x;
-    const int J1 = a.J + 1;
-    // this thread's Gram entries e = tid + k * 256 as (row, col), row <= col
-    int er[kZPer], ec[kZPer];
-    double acc[kZPer];
-#pragma unroll
-    for (int k = 0; k < kZPer; k++) {
-        const int e = tid + k * kZThreads;
-        int r = 0, rem = e;
-        while (r < J1 && rem >= J1 - r) { rem -= J1 - r; r++; }
-        er[k] = r;
-        ec[k] = r + rem;
-        acc[k] = 0.0;
+    const int wave = tid >> 6, lane = tid & 63;
+    const int li = lane & 15, lk = lane >> 4;
+    const int J = a.J, J1 = J + 1, M = a.M;
+    const int ksteps = (M + 3) / 4;
+    // the table, zero past J and M; monomial rows past M and basis columns
+    // past J stay zero for every chunk
+    for (int e = tid; e < MP * JP; e += kZThreads) {
+        const int m = e / JP, j = e % JP;
+        ct[m][j] = (j < J && m < M) ? a.coef[(int64_t)j * M + m] : 0.0;
     }
-    for (int64_t base = (int64_t)blockIdx.x * kZChunk; base < a.n;
-         base += (int64_t)gridDim.x * kZChunk) {
-        const int64_t i = base + tid;
-        const bool ok = tid < kZChunk && i < a.n && !(a.mask && a.mask[i]);
-        // a chunk whose rays are all masked adds nothing (outside the pupil)
-        if (!__syncthreads_or(ok)) continue;
+    for (int e = tid; e < (MP - M) * kZChunk; e += kZThreads)
+        mono[M + e / kZChunk][e % kZChunk] = 0.0;
+    for (int e = tid; e < kZChunk * (JP - J1); e += kZThreads)
+        basis[e / (JP - J1)][J1 + e % (JP - J1)] = 0.0;
+    v4d g[NBLK];
+    for (int b = 0; b < NBLK; b++) g[b] = (v4d){0.0, 0.0, 0.0, 0.0};
+    const int p0 = wave * 16;                        // this wave's rays in the chunk
+    // the inputs of a chunk's rays (threads 0..63), loaded one chunk ahead
+    // of their use so the loads overlap the previous chunk's work
+    struct In { double x, y, t, f; int keep; };
+    auto fetch = [&](int64_t b, In& r) {
+        const int64_t i = b + tid;
+        r.keep = 0;
+        r.x = r.y = r.t = r.f = 0.0;
+        if (tid < kZChunk && i < a.n) {
+            r.x = a.x[i];
+            r.y = a.y[i];
+            r.t = a.t[i];
+            r.f = a.flux ? a.flux[i] : 1.0;
+            r.keep = a.mask ? (a.mask[i] == 0) : 1;
+        }
+    };
+    const int64_t stride = (int64_t)gridDim.x * kZChunk;
+    In cur, nxt;
+    fetch((int64_t)blockIdx.x * kZChunk, cur);
+    for (int64_t base = (int64_t)blockIdx.x * kZChunk; base < a.n; base += stride) {
+        fetch(base + stride, nxt);
+        const bool ok = cur.keep != 0;
+        // a chunk whose rays are all masked adds nothing (outside the pupil);
+        // the barrier also orders the previous chunk's reads before these writes
+        if (!__syncthreads_or(ok)) {
+            cur = nxt;
+            continue;
+        }
         if (tid < kZChunk) {
             double u = 0.0, v = 0.0, w = 0.0;
             if (ok) {
-                u = a.x[i] * a.inv_r;
-                v = a.y[i] * a.inv_r;
-                w = (a.t0 - a.t[i]) / a.wavelength;      // analysis.py:375, 639
-                if (a.flux) w = w * a.flux[i];
+                u = cur.x * a.inv_r;
+                v = cur.y * a.inv_r;
+                w = (a.t0 - cur.t) / a.wavelength;       // analysis.py:375, 639
+                if (a.flux) w = w * cur.f;
             }
             double up[kZMaxN + 1], vp[kZMaxN + 1];
             up[0] = ok ? 1.0 : 0.0;
@@ -96,49 +144,71 @@ __global__ __launch_bounds__(kZThreads) void k_zernike_gram(ZArgs a)
                         mono[d * (d + 1) / 2 + q][tid] = up[d - q] * vp[q];
                 }
             }
-            basis[a.J][tid] = w;
+            basis[tid][J] = w;
         }
         __syncthreads();
-        for (int e = tid; e < a.J * kZChunk; e += kZThreads) {
-            const int j = e / kZChunk, p = e % kZChunk;
-            const double* cj = a.coef + (int64_t)j * a.M;
-            // four interleaved partial sums (fixed order) shorten the
-            // dependent FMA chain
-            double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-            int m = 0;
-            for (; m + 4 <= a.M; m += 4) {
-                s0 += cj[m] * mono[m][p];
-                s1 += cj[m + 1] * mono[m + 1][p];
-                s2 += cj[m + 2] * mono[m + 2][p];
-                s3 += cj[m + 3] * mono[m + 3][p];
+        // 2. basis values of this wave's 16 rays, every 16-column block of C
+        // at once (independent accumulators interleaved)
+        v4d z[NB];
+#pragma unroll
+        for (int jb = 0; jb < NB; jb++) z[jb] = (v4d){0.0, 0.0, 0.0, 0.0};
+        for (int ks = 0; ks < ksteps; ks++) {
+            const int m = ks * 4 + lk;
+            const double am = mono[m][p0 + li];
+#pragma unroll
+            for (int jb = 0; jb < NB; jb++)
+                z[jb] = __builtin_amdgcn_mfma_f64_16x16x4f64(am, ct[m][jb * 16 + li], z[jb], 0, 0,
+                                                             0);
+        }
+#pragma unroll
+        for (int jb = 0; jb < NB; jb++) {
+            const int j = jb * 16 + li;
+            if (j < J) {
+#pragma unroll
+                for (int e = 0; e < 4; e++) basis[p0 + 4 * e + lk][j] = z[jb][e];
             }
-            for (; m < a.M; m++) s0 += cj[m] * mono[m][p];
-            basis[j][p] = (s0 + s1) + (s2 + s3);
         }
-        __syncthreads();
+        // 3. Gram blocks over the same 16 rays, 4 rays per step; the lane's
+        // value for block row/column x is basis[ray][16 x + li]
 #pragma unroll
-        for (int k = 0; k < kZPer; k++) {
-            if (tid + k * kZThreads < a.nent) {
-                const double* br = basis[er[k]];
-                const double* bc = basis[ec[k]];
-                double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+        for (int ks = 0; ks < 4; ks++) {
+            const int p = p0 + ks * 4 + lk;
+            double xv[NB];
 #pragma unroll
-                for (int p = 0; p < kZChunk; p += 4) {
-                    s0 += br[p] * bc[p];
-                    s1 += br[p + 1] * bc[p + 1];
-                    s2 += br[p + 2] * bc[p + 2];
-                    s3 += br[p + 3] * bc[p + 3];
+            for (int x = 0; x < NB; x++) xv[x] = basis[p][x * 16 + li];
+            int b = 0;
+#pragma unroll
+            for (int rb = 0; rb < NB; rb++)
+#pragma unroll
+                for (int cb = rb; cb < NB; cb++, b++)
+                    g[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(xv[rb], xv[cb], g[b], 0, 0, 0);
+        }
+        cur = nxt;
+    }
+    // the four waves' blocks added in wave order, then the upper triangle out
+    static_assert(NBLK * 256 <= MP * kZChunk, "the wave sums fit in the monomial buffer");
+    double* sum = &mono[0][0];                       // NBLK x 256 doubles
+    __syncthreads();
+    for (int w = 0; w < kZThreads / 64; w++) {
+        if (wave == w) {
+#pragma unroll
+            for (int b = 0; b < NBLK; b++)
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    double& d = sum[b * 256 + (4 * e + lk) * 16 + li];
+                    d = (w == 0) ? g[b][e] : d + g[b][e];
                 }
-                acc[k] += (s0 + s1) + (s2 + s3);
-            }
         }
         __syncthreads();
     }
     double* out = a.partial + (int64_t)blockIdx.x * a.nent;
-#pragma unroll
-    for (int k = 0; k < kZPer; k++) {
-        const int e = tid + k * kZThreads;
-        if (e < a.nent) out[e] = acc[k];
+    for (int e = tid; e < a.nent; e += kZThreads) {
+        int r = 0, rem = e;
+        while (rem >= J1 - r) { rem -= J1 - r; r++; }
+        const int c = r + rem;
+        const int rb = r >> 4, cb = c >> 4;
+        const int b = rb * NB - rb * (rb - 1) / 2 + (cb - rb);   // block (rb, cb), rb <= cb
+        out[e] = sum[b * 256 + (r & 15) * 16 + (c & 15)];
     }
 }
 
@@ -146,7 +216,7 @@ __global__ __launch_bounds__(kZThreads) void k_zernike_gram(ZArgs a)
 // block covers kZRedEnt consecutive entries (coalesced reads) with
 // kZRedSlices lanes per entry, lane g summing b = g, g + kZRedSlices, ...;
 // the slice sums are then added in g order.
-constexpr int kZRedEnt = 32;
+constexpr int kZRedEnt = 8;
 constexpr int kZRedSlices = 256 / kZRedEnt;
 
 __global__ __launch_bounds__(256) void k_zernike_reduce(const double* __restrict__ partial,
@@ -206,7 +276,16 @@ int bt_zernike_gram(const double* x, const double* y, const double* t, const dou
     a.nent = (jmax + 1) * (jmax + 2) / 2;
     a.partial = static_cast<double*>(workspace);
     hipStream_t s = as_stream(stream);
-    hipLaunchKernelGGL(k_zernike_gram, dim3(kZGrid), dim3(kZThreads), 0, s, a);
+    const int nb = (jmax + 1 + 15) / 16;
+    if ((a.M + 3) / 4 > zernike_ksteps(nb))
+        return fail(BT_E_SIZE, "bt_zernike_gram: nmax too large for jmax");
+    switch (nb) {
+    case 1: hipLaunchKernelGGL(k_zernike_gram<1>, dim3(kZGrid), dim3(kZThreads), 0, s, a); break;
+    case 2: hipLaunchKernelGGL(k_zernike_gram<2>, dim3(kZGrid), dim3(kZThreads), 0, s, a); break;
+    case 3: hipLaunchKernelGGL(k_zernike_gram<3>, dim3(kZGrid), dim3(kZThreads), 0, s, a); break;
+    case 4: hipLaunchKernelGGL(k_zernike_gram<4>, dim3(kZGrid), dim3(kZThreads), 0, s, a); break;
+    default: hipLaunchKernelGGL(k_zernike_gram<5>, dim3(kZGrid), dim3(kZThreads), 0, s, a); break;
+    }
     int rc = check_launch("bt_zernike_gram");
     if (rc) return rc;
     hipLaunchKernelGGL(k_zernike_reduce, dim3((a.nent + kZRedEnt - 1) / kZRedEnt), dim3(256), 0,
