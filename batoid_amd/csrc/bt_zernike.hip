@@ -276,9 +276,10 @@ int bt_zernike_gram(const double* x, const double* y, const double* t, const dou
     a.nent = (jmax + 1) * (jmax + 2) / 2;
     a.partial = static_cast<double*>(workspace);
     hipStream_t s = as_stream(stream);
-    const int nb = (jmax + 1 + 15) / 16;
-    if ((a.M + 3) / 4 > zernike_ksteps(nb))
-        return fail(BT_E_SIZE, "bt_zernike_gram: nmax too large for jmax");
+    // Gram blocks per side for jmax, raised if the table has more monomials
+    // than that variant's K steps hold (a table of higher order than jmax needs)
+    int nb = (jmax + 1 + 15) / 16;
+    while (nb < 5 && zernike_ksteps(nb) < (a.M + 3) / 4) nb++;
     switch (nb) {
     case 1: hipLaunchKernelGGL(k_zernike_gram<1>, dim3(kZGrid), dim3(kZThreads), 0, s, a); break;
     case 2: hipLaunchKernelGGL(k_zernike_gram<2>, dim3(kZGrid), dim3(kZThreads), 0, s, a); break;

@@ -241,3 +241,34 @@ def test_huygensPSF_vs_zemax_hsc():
     assert abs(zg1 - bg1) < 0.01
     assert abs(zg2 - bg2) < 1e-3
     assert abs(zs - bs) < 0.1
+
+
+def test_gram_table_of_higher_order():
+    """A coefficient table with more monomial columns than jmax needs (zero
+    there) gives the same Gram matrix, bit for bit: the C-ABI takes any
+    nmax <= 11 for any jmax."""
+    from batoid_amd import _lib
+    rng = np.random.default_rng(3)
+    n = 5000
+    x, y = rng.uniform(-1, 1, n), rng.uniform(-1, 1, n)
+    t = rng.normal(0, 1e-6, n)
+    lib = _lib.load()
+    d = _device.f64
+    outs = []
+    for jmax in (4, 22):
+        C, nmax = zernike_table(jmax)
+        for nm in (nmax, 11):
+            M = (nm + 1) * (nm + 2) // 2
+            Cp = np.zeros((jmax, M))
+            Cp[:, :C.shape[1]] = C
+            ws = torch.empty(int(lib.bt_zernike_workspace_bytes(jmax)), dtype=torch.uint8,
+                             device=_device.device())
+            out = torch.empty((jmax + 1) * (jmax + 2) // 2, dtype=torch.float64,
+                              device=ws.device)
+            xs, ys, ts, cs = d(x), d(y), d(t), d(Cp)
+            _lib.check(lib.bt_zernike_gram(_device.ptr(xs), _device.ptr(ys), _device.ptr(ts),
+                                           None, None, n, 0.0, 1e-6, 1.0, _device.ptr(cs),
+                                           jmax, nm, _device.ptr(out), _device.ptr(ws),
+                                           ws.numel(), _device.stream()), "gram")
+            outs.append(out.cpu().numpy())
+        np.testing.assert_array_equal(outs[-2], outs[-1])
