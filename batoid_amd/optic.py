@@ -178,6 +178,11 @@ class Interface(Optic):
             rForward, rReverse = rReverse, rForward
         return rForward, rReverse
 
+    def _split_batch(self, inputs, minFlux, _verbose=False):
+        """traceSplit of several (rays, reverse) inputs (no host
+        synchronisation: the launches queue on the stream)."""
+        return [self.traceSplit(rv, minFlux, rev, _verbose) for rv, rev in inputs]
+
     @cached_property
     def R_outer(self):
         obsc = self.obscuration
@@ -491,45 +496,66 @@ class CompoundOptic(Optic):
         parts (optic.py:1184-1282); rays that are vignetted or fall below
         ``minFlux`` are dropped after each item (on the GPU: an
         order-preserving compaction).  Returns (forward, reverse) lists of
-        RayVectors, each with its ``path``."""
+        RayVectors, each with its ``path``, in the reference's order."""
         if _verbose:
             s = "reverse" if reverse else "forward"
             print("traceSplit {}       {:30s} flux = {:18.8f}   nphot = {:10d}".format(
                 s, self.name, float(rv.torch("flux").sum()), len(rv)))
+        return self._split_batch([(rv, reverse)], minFlux, _verbose)[0]
+
+    def _split_batch(self, inputs, minFlux, _verbose=False):
+        """traceSplit of several (rays, reverse) inputs at once.
+
+        The reference pops one work item at a time from a LIFO stack and
+        reads back the counts of its outputs before the next.  Here every
+        pending item is processed in one round (items at the same index
+        together, so a nested CompoundOptic also takes them in one call),
+        with one read-back of the counts of all their outputs; each item
+        carries its place in the reference's depth-first order -- the
+        parent's key plus -p for its p-th pushed child (the last pushed is
+        popped first) -- and the outputs, appended while their parent is
+        processed, are sorted by (parent key, position) at the end, which
+        reproduces the reference's lists exactly."""
         if self.skip:
-            return (None, rv) if reverse else (rv, None)
-        work = [(rv, "forward", 0)] if not reverse else [(rv, "reverse", len(self.items) - 1)]
-        outForward, outReverse = [], []
+            return [((None, rv) if rev else (rv, None)) for rv, rev in inputs]
+        last = len(self.items) - 1
+        # work entries: (input, key, rays, reverse, item index)
+        work = [(k, (), rv, rev, last if rev else 0) for k, (rv, rev) in enumerate(inputs)]
+        outs = [([], []) for _ in inputs]              # per input: [(order, rays)] x 2
         while work:
-            rays, direction, index = work.pop()
-            item = self.items[index]
-            rF, rR = item.traceSplit(rays, minFlux=minFlux, reverse=(direction == "reverse"),
-                                     _verbose=_verbose)
-            # the surviving rays of every output, filtered together (one
-            # host synchronisation for all their counts)
-            outs = []
-            for side, rl in enumerate((rF, rR)):
-                if not isinstance(rl, list):           # a skipped item passes rays on
-                    rl = [rl]
-                outs += [(side, rr) for rr in rl if rr]
-            sels = RayVector._select_many([rr for _, rr in outs], minFlux)
-            lists = [[], []]
-            for (side, rr), sel in zip(outs, sels):
+            groups = {}
+            for w in work:
+                groups.setdefault(w[4], []).append(w)
+            work = []
+            children = []                              # (entry, side, rays)
+            for index in sorted(groups):
+                entries = groups[index]
+                results = self.items[index]._split_batch(
+                    [(w[2], w[3]) for w in entries], minFlux, _verbose)
+                for w, (rF, rR) in zip(entries, results):
+                    for side, rl in enumerate((rF, rR)):
+                        if not isinstance(rl, list):   # a skipped item passes rays on
+                            rl = [rl]
+                        children += [(w, side, rr) for rr in rl if rr]
+            sels = RayVector._select_many([c[2] for c in children], minFlux)
+            pushed, seq = {}, {}
+            for (w, side, rr), sel in zip(children, sels):
                 sel.path = rr.path
-                lists[side].append(sel)
-            for rr in lists[0]:
-                if len(rr) > 0:
-                    if index == len(self.items) - 1:
-                        outForward.append(rr)
-                    else:
-                        work.append((rr, "forward", index + 1))
-            for rr in lists[1]:
-                if len(rr) > 0:
-                    if index == 0:
-                        outReverse.append(rr)
-                    else:
-                        work.append((rr, "reverse", index - 1))
-        return outForward, outReverse
+                if len(sel) == 0:
+                    continue
+                k, key, _, _, index = w
+                parent = id(w)
+                if (side == 0 and index == last) or (side == 1 and index == 0):
+                    s = seq.get(parent, 0)
+                    seq[parent] = s + 1
+                    outs[k][side].append(((key, s), sel))
+                else:
+                    p = pushed.get(parent, 0)
+                    pushed[parent] = p + 1
+                    work.append((k, key + (-p,), sel, side == 1,
+                                 index + 1 if side == 0 else index - 1))
+        return [tuple([r for _, r in sorted(lst, key=lambda e: e[0])] for lst in o)
+                for o in outs]
 
     def clearObscuration(self, unless=()):
         for item in self.items:

@@ -59,6 +59,7 @@ class RayVector:
             cols[name] = _device.u8(np.broadcast_to(np.asarray(val, bool), shape).ravel())
         self._cols = cols
         self._buf = None
+        self._rays_c = None
         self.coordSys = coordSys
 
     @staticmethod
@@ -68,6 +69,7 @@ class RayVector:
         ret = RayVector.__new__(RayVector)
         ret._cols = cols
         ret._buf = buf
+        ret._rays_c = None
         ret.coordSys = coordSys
         return ret
 
@@ -152,10 +154,15 @@ class RayVector:
     # ---- C-ABI view -------------------------------------------------------
     @property
     def _rays(self):
-        r = _lib.BtRays()
-        for k in _COLS:
-            setattr(r, k, self._cols[k].data_ptr())
-        r.n = len(self)
+        """The bt_rays descriptor of the columns (built once: the column
+        tensors of a RayVector are never replaced, only written in place)."""
+        r = self.__dict__.get("_rays_c")
+        if r is None:
+            r = _lib.BtRays()
+            for k in _COLS:
+                setattr(r, k, self._cols[k].data_ptr())
+            r.n = len(self)
+            self._rays_c = r
         return r
 
     def torch(self, name):
@@ -301,6 +308,7 @@ class RayVector:
         self._cols = {k: (_device.u8(a) if k in _BCOLS else _device.f64(a))
                       for k, a in zip(_COLS, args[:-1])}
         self._buf = None
+        self._rays_c = None
         self.coordSys = args[-1]
 
     # ---- generators ----------------------------------------------------------
