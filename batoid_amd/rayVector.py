@@ -74,21 +74,47 @@ class RayVector:
         return ret
 
     @staticmethod
-    def _empty(n, coordSys):
-        """n rays of uninitialised device columns, views of one allocation
-        (each column starting on a 256-byte boundary)."""
-        n = int(n)
-        sf = -(-n // 32) * 32                  # float64 column stride, elements
+    def _layout(n):
+        """Byte offsets of the columns of n rays in one allocation (each
+        column starting on a 256-byte boundary) and the allocation's size."""
+        sf = -(-n // 32) * 32 * 8              # float64 column stride, bytes
         sb = -(-n // 256) * 256                # uint8 column stride, bytes
         nf = len(_FCOLS)
-        buf = torch.empty(nf * sf * 8 + len(_BCOLS) * sb, dtype=torch.uint8,
-                          device=_device.device())
-        f = buf[: nf * sf * 8].view(torch.float64)
-        cols = {k: f[i * sf: i * sf + n] for i, k in enumerate(_FCOLS)}
-        base = nf * sf * 8
-        for j, k in enumerate(_BCOLS):
-            cols[k] = buf[base + j * sb: base + j * sb + n]
-        return RayVector._directInit(cols, coordSys, buf)
+        offs = [i * sf for i in range(nf)] + [nf * sf + j * sb for j in range(len(_BCOLS))]
+        return offs, nf * sf + len(_BCOLS) * sb
+
+    @staticmethod
+    def _empty(n, coordSys):
+        """n rays of uninitialised device columns in one allocation.  The
+        column tensors (views) are made on first use; the library reads the
+        columns through pointers computed from the layout."""
+        n = int(n)
+        offs, nbytes = RayVector._layout(n)
+        buf = torch.empty(nbytes, dtype=torch.uint8, device=_device.device())
+        ret = RayVector._directInit(None, coordSys, buf)
+        ret._n = n
+        return ret
+
+    @property
+    def _cols(self):
+        cols = self.__dict__.get("_cols_d")
+        if cols is None:                      # views of the single allocation
+            n, buf = self._n, self._buf
+            offs, _ = RayVector._layout(n)
+            cols = {}
+            for k, o in zip(_COLS, offs):
+                if k in _BCOLS:
+                    cols[k] = buf[o: o + n]
+                else:
+                    cols[k] = buf[o: o + 8 * n].view(torch.float64)
+            self.__dict__["_cols_d"] = cols
+        return cols
+
+    @_cols.setter
+    def _cols(self, cols):
+        self.__dict__["_cols_d"] = cols
+        if cols is not None:
+            self._n = cols["t"].numel()
 
     def _select(self, minFlux):
         """The rays that are not vignetted and have flux >= minFlux, in order,
@@ -159,9 +185,15 @@ class RayVector:
         r = self.__dict__.get("_rays_c")
         if r is None:
             r = _lib.BtRays()
-            for k in _COLS:
-                setattr(r, k, self._cols[k].data_ptr())
-            r.n = len(self)
+            if self.__dict__.get("_cols_d") is None:      # from the layout
+                base = self._buf.data_ptr()
+                offs, _ = RayVector._layout(self._n)
+                for k, o in zip(_COLS, offs):
+                    setattr(r, k, base + o)
+            else:
+                for k in _COLS:
+                    setattr(r, k, self._cols[k].data_ptr())
+            r.n = self._n
             self._rays_c = r
         return r
 
@@ -170,7 +202,7 @@ class RayVector:
         return self._cols[name]
 
     def __len__(self):
-        return self._cols["t"].numel()
+        return self._n
 
     # ---- host accessors ---------------------------------------------------
     def _host(self, name):
