@@ -22,7 +22,8 @@ namespace bt {
 
 __global__ __launch_bounds__(kBlock) void k_rsplit(
     Xform xf, const bt_surface* __restrict__ surf, const bt_medium* __restrict__ m2,
-    double reflect, double transmit, bt_rays rv, bt_rays sp, int niter)
+    double reflect, double transmit, const bt_obsc* __restrict__ obsc, int olen, bt_rays rv,
+    bt_rays sp, int niter)
 {                                                    // batoid.cpp:582-672
     const int64_t i = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (i >= rv.n) return;
@@ -35,6 +36,9 @@ __global__ __launch_bounds__(kBlock) void k_rsplit(
     // the split ray starts as a copy of the input (trace.py:125)
     double sx = x0, sy = y0, sz = z0, svx = vx0, svy = vy0, svz = vz0, st = t0, sf = f;
     bool svig = vig0, sfail = fail0;
+    // the refracted ray's final position and flag, for the obscuration
+    double rx = x0, ry = y0;
+    bool rvig = vig0;
     if (!fail0) {
         double dx = x0-dr[0];
         double dy = y0-dr[1];
@@ -92,11 +96,14 @@ __global__ __launch_bounds__(kBlock) void k_rsplit(
             rv.y[i] = y;
             rv.z[i] = z;
             rv.t[i] = t;
+            rx = x;
+            ry = y;
             if (sinsqr > 1.0) {                      // total internal reflection
                 const double nan = __builtin_nan("");
                 rv.vx[i] = nan; rv.vy[i] = nan; rv.vz[i] = nan;
                 rv.failed[i] = 1;
                 rv.vignetted[i] = 1;
+                rvig = true;
             } else {
                 const double nfactor = eta*alpha + sqrt(1-sinsqr);
                 double ux = eta*nvx - nfactor*nx;
@@ -110,9 +117,16 @@ __global__ __launch_bounds__(kBlock) void k_rsplit(
         } else {
             rv.failed[i] = 1;
             rv.vignetted[i] = 1;
+            rvig = true;
             svig = true;
             sfail = true;
         }
+    }
+    // the Interface's obscuration on both parts (optic.py:491-494), every
+    // ray included, as batoid.cpp:151-172
+    if (olen > 0) {
+        if (!rvig && obsc_contains(obsc, olen, rx, ry)) rv.vignetted[i] = 1;
+        if (!svig) svig = obsc_contains(obsc, olen, sx, sy);
     }
     sp.x[i] = sx; sp.y[i] = sy; sp.z[i] = sz;
     sp.vx[i] = svx; sp.vy[i] = svy; sp.vz[i] = svz;
@@ -231,11 +245,13 @@ using namespace bt;
 extern "C" {
 
 int bt_rsplit(const bt_surface* surface, const double* dr, const double* drot,
-              const bt_medium* m2, const bt_coating* coating, bt_rays rays,
-              bt_rays split, int niter, void* stream)
+              const bt_medium* m2, const bt_coating* coating, const bt_obsc* obsc,
+              int obsc_len, bt_rays rays, bt_rays split, int niter, void* stream)
 {
     if (!surface || !dr || !drot || !m2 || !coating)
         return fail(BT_E_NULL, "bt_rsplit: null argument");
+    if (obsc_len < 0 || (obsc_len > 0 && !obsc))
+        return fail(BT_E_SIZE, "bt_rsplit: bad obscuration program");
     if (niter < 0) return fail(BT_E_SIZE, "bt_rsplit: negative niter");
     if (rays.n < 0 || split.n < rays.n)
         return fail(BT_E_SIZE, "bt_rsplit: split holds fewer rays");
@@ -244,7 +260,7 @@ int bt_rsplit(const bt_surface* surface, const double* dr, const double* drot,
         return fail(BT_E_NULL, "bt_rsplit: incomplete ray columns");
     hipLaunchKernelGGL(k_rsplit, dim3(grid_for(rays.n)), dim3(kBlock), 0, as_stream(stream),
                        make_xform(dr, drot), surface, m2, coating->reflectivity,
-                       coating->transmissivity, rays, split, niter);
+                       coating->transmissivity, obsc, obsc_len, rays, split, niter);
     return check_launch("bt_rsplit");
 }
 

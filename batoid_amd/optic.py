@@ -161,12 +161,8 @@ class Interface(Optic):
                 s, self.name, float(rv.torch("flux").sum()), len(rv)))
         if self.skip:
             return (None, rv) if reverse else (rv, None)
-        refracted, reflected = self.rSplit(rv, reverse=reverse)
-        # the obscuration applies to both directions
-        if self.obscuration is not None:
-            for r in (refracted, reflected):
-                if r is not None:
-                    self.obscuration.obscure(r)
+        # the obscuration applies to both directions, in the same pass
+        refracted, reflected = self.rSplit(rv, reverse=reverse, obscuration=self.obscuration)
         path = list(getattr(rv, 'path', [])) + [self.name]
         for r in (refracted, reflected):
             if r is not None:
@@ -248,14 +244,15 @@ class RefractiveInterface(Interface):
         self.forwardCoating = SimpleCoating(reflectivity=0.0, transmissivity=1.0)
         self.reverseCoating = SimpleCoating(reflectivity=0.0, transmissivity=1.0)
 
-    def rSplit(self, rv, reverse=False):
+    def rSplit(self, rv, reverse=False, obscuration=None):
         """(refracted, reflected); as optic.py:786-797, the forward direction
         uses reverseCoating and the reverse direction forwardCoating."""
         if reverse:
             m1, m2, coating = self.outMedium, self.inMedium, self.forwardCoating
         else:
             m1, m2, coating = self.inMedium, self.outMedium, self.reverseCoating
-        return self.surface.rSplit(rv, m1, m2, coating, coordSys=self.coordSys)
+        return self.surface.rSplit(rv, m1, m2, coating, coordSys=self.coordSys,
+                                   obscuration=obscuration)
 
     def _interact(self, rv, reverse, obscuration):
         from .trace import refract
@@ -274,9 +271,9 @@ class Mirror(Interface):
         self.forwardCoating = SimpleCoating(reflectivity=1.0, transmissivity=0.0)
         self.reverseCoating = SimpleCoating(reflectivity=1.0, transmissivity=0.0)
 
-    def rSplit(self, rv, reverse=False):
+    def rSplit(self, rv, reverse=False, obscuration=None):
         from .trace import reflect
-        return None, reflect(self.surface, rv, self.coordSys)
+        return None, reflect(self.surface, rv, self.coordSys, obscuration=obscuration)
 
     def _interact(self, rv, reverse, obscuration):
         from .trace import reflect
@@ -294,10 +291,10 @@ class Detector(Interface):
         self.forwardCoating = SimpleCoating(reflectivity=0.0, transmissivity=1.0)
         self.reverseCoating = None
 
-    def rSplit(self, rv, reverse=False):
+    def rSplit(self, rv, reverse=False, obscuration=None):
         assert not reverse
         return self.surface.rSplit(rv, self.inMedium, self.outMedium, self.forwardCoating,
-                                   coordSys=self.coordSys)
+                                   coordSys=self.coordSys, obscuration=obscuration)
 
 
 class Baffle(Interface):
@@ -308,9 +305,9 @@ class Baffle(Interface):
         self.forwardCoating = SimpleCoating(reflectivity=0.0, transmissivity=1.0)
         self.reverseCoating = SimpleCoating(reflectivity=0.0, transmissivity=1.0)
 
-    def rSplit(self, rv, reverse=False):
+    def rSplit(self, rv, reverse=False, obscuration=None):
         from .trace import intersect
-        return intersect(self.surface, rv, self.coordSys), None
+        return intersect(self.surface, rv, self.coordSys, obscuration=obscuration), None
 
 
 class OPDScreen(Interface):
@@ -324,7 +321,7 @@ class OPDScreen(Interface):
     def _interact(self, rv, reverse, obscuration):
         raise NotImplementedError("OPDScreen is outside the GPU trace path")
 
-    def rSplit(self, rv, reverse=False):
+    def rSplit(self, rv, reverse=False, obscuration=None):
         raise NotImplementedError("OPDScreen is outside the GPU trace path")
 
 

@@ -92,11 +92,11 @@ class Surface(DeviceCached):
         from .trace import reflect
         return reflect(self, rv, coordSys, coating)
 
-    def rSplit(self, rv, inMedium, outMedium, coating, coordSys=None):
+    def rSplit(self, rv, inMedium, outMedium, coating, coordSys=None, obscuration=None):
         """Refracted (``rv``, in place) and reflected (new) rays at this
         surface, fluxes split by ``coating`` (surface.py:107-133)."""
         from .trace import rSplit
-        return rSplit(self, rv, inMedium, outMedium, coating, coordSys)
+        return rSplit(self, rv, inMedium, outMedium, coating, coordSys, obscuration)
 
     def refract(self, rv, inMedium, outMedium, coordSys=None, coating=None):
         """Intersect and refract rays in place (trace.py:101-116)."""

@@ -111,18 +111,20 @@ def refract(surface, rv, m1, m2, coordSys=None, coating=None, obscuration=None):
     return rv
 
 
-def rSplit(surface, rv, inMedium, outMedium, coating, coordSys=None):
+def rSplit(surface, rv, inMedium, outMedium, coating, coordSys=None, obscuration=None):
     """Split rays at ``surface`` into refracted (``rv``, in place) and
     reflected (a new RayVector) parts with the coating's flux fractions
-    (trace.py:119-136, batoid.cpp:509-674).  Returns (refracted, reflected)."""
+    (trace.py:119-136, batoid.cpp:509-674).  ``obscuration``, if given, is
+    applied to both parts in the same pass.  Returns (refracted, reflected)."""
     from .rayVector import RayVector
     if coordSys is None:
         coordSys = rv.coordSys
     ct = CoordTransform(rv.coordSys, coordSys)
     split = RayVector._empty(len(rv), coordSys)
+    optr, olen = _obsc_args(obscuration)
     _lib.call("bt_rsplit", surface._dev.ptr(), _lib.dvec(ct.dr), _lib.dvec(ct.drot.ravel()),
-              outMedium._dev.ptr(), _coating_ptr(coating), rv._rays, split._rays, _niter(),
-              _device.stream())
+              outMedium._dev.ptr(), _coating_ptr(coating), optr, olen, rv._rays, split._rays,
+              _niter(), _device.stream())
     rv.coordSys = coordSys
     return rv, split
 

@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define BT_ABI_VERSION 4
+#define BT_ABI_VERSION 5
 
 /* ---- status codes ------------------------------------------------------ */
 #define BT_OK            0
@@ -327,10 +327,13 @@ int bt_refract(const bt_surface* surface, const double* dr,
  * (written to `split`, flux x reflectivity).  Rays already failed on entry
  * are copied to `split` unchanged (the reference's rvSplit starts as a copy
  * of rv, batoid/trace.py:125); rays that miss the surface are
- * marked failed and vignetted in both.  `split` must hold rays.n rays. */
+ * marked failed and vignetted in both.  `split` must hold rays.n rays.
+ * `obsc`/`obsc_len` optionally fold the Interface's obscuration into the
+ * same pass for both parts (optic.py:491-494, as traceSplit applies it
+ * after rSplit); pass NULL/0 for the plain reference call.                */
 int bt_rsplit(const bt_surface* surface, const double* dr, const double* drot,
-              const bt_medium* m2, const bt_coating* coating, bt_rays rays,
-              bt_rays split, int niter, void* stream);
+              const bt_medium* m2, const bt_coating* coating, const bt_obsc* obsc,
+              int obsc_len, bt_rays rays, bt_rays split, int niter, void* stream);
 
 /* ---- ray selection ------------------------------------------------------
  * The ray filter of CompoundOptic.traceSplit (optic.py:1256-1266): keep the

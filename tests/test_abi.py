@@ -42,4 +42,4 @@ def test_struct_layouts_match():
     sizes = (ctypes.c_int64 * len(_lib.STRUCTS))()
     assert lib.bt_struct_sizes(sizes, len(_lib.STRUCTS)) == 0
     assert list(sizes) == [ctypes.sizeof(s) for s in _lib.STRUCTS]
-    assert lib.bt_abi_version() == 4
+    assert lib.bt_abi_version() == 5

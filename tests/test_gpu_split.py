@@ -77,6 +77,31 @@ def test_coated_and_rsplit_bit_exact(name, surf):
         assert refr.coordSys == cs and refl.coordSys == cs
 
 
+@pytest.mark.parametrize("name,surf", surfaces()[:4], ids=[s[0] for s in surfaces()[:4]])
+def test_rsplit_with_obscuration_bit_exact(name, surf):
+    """bt_rsplit with the obscuration folded in equals rSplit followed by
+    the obscuration of both parts (optic.py:491-494), bit for bit."""
+    O = oracle.load()
+    tab = host_records(surf)
+    med = media()[2][1]
+    mtab = host_records(med)
+    coat = bt.SimpleCoating(0.0371, 0.9588)
+    obsc = bt.ObscNegation(bt.ObscAnnulus(0.2, 0.8))
+    otab = host_records(obsc)
+    for tname, (dr, drot) in transforms():
+        cs = _frame(dr, drot)
+        base = HostRays(**random_rays(N, seed=23, upward=0.3, spread=0.3))
+        a, sa = base.copy(), base.copy()
+        O.or_rsplit(tab.ptr(), _dv(dr), _dv(drot), mtab.ptr(), coat.reflectivity,
+                    coat.transmissivity, a.struct, sa.struct, 5, nthreads())
+        O.or_obscure(otab.ptr(), otab.count, a.struct, nthreads())
+        O.or_obscure(otab.ptr(), otab.count, sa.struct, nthreads())
+        refr, refl = surf.rSplit(_gpu(base), bt.ConstMedium(1.0), med, coat, coordSys=cs,
+                                 obscuration=obsc)
+        assert_cols_identical(_host(refr), a.cols, what=f"{name}/rsplit+obsc refracted/{tname}")
+        assert_cols_identical(_host(refl), sa.cols, what=f"{name}/rsplit+obsc reflected/{tname}")
+
+
 @pytest.mark.parametrize("n", [0, 1, 255, 256, 257, 100_003, 2_000_000])
 def test_select_matches_numpy(n):
     rng = np.random.default_rng(n)
