@@ -231,14 +231,24 @@ __device__ __forceinline__ void mlp_features(const bt_step* __restrict__ st,
     f[5] = (float)vz;
 }
 
-// Waves per SIMD: mode 0 is held to four (some registers spill to scratch;
-// the extra wave hides more FP64 latency: 8 % faster on the TMA, 9-12 % on
-// the Newton-heavy optics), mode 2 to three (left alone the compiler would
-// settle for two); mode 1 keeps the compiler's three, where four measured
-// slower on the TMA.
+// Waves per SIMD: the analytic-kinds kernels of modes 0 and 1 are held to
+// five (95 VGPRs; one 8-byte value per ray goes to scratch), which the
+// compiler would otherwise leave at four (97 VGPRs); the all-kinds kernel
+// of mode 0 is held to four (some registers spill to scratch; the extra
+// wave hides more FP64 latency: 8 % faster on the TMA, 9-12 % on the
+// Newton-heavy optics), mode 2 to three (left alone the compiler would
+// settle for two); the all-kinds kernel of mode 1 keeps the compiler's
+// three, where four measured slower on the TMA.
+template <int Mode, int K>
+constexpr int plan_waves_per_simd()
+{
+    if (K == kKindsAnalytic && Mode < 2) return 5;
+    return Mode == 0 ? 4 : (Mode == 2 ? 3 : 1);
+}
+
 template <int Mode, int K>
 __global__ __launch_bounds__(kBlock)
-__attribute__((amdgpu_waves_per_eu(Mode == 0 ? 4 : (Mode == 2 ? 3 : 1))))
+__attribute__((amdgpu_waves_per_eu(plan_waves_per_simd<Mode, K>())))
 void k_plan(
     bt_plan plan, bt_rays in, bt_rays out, int niter, int need_w)
 {
