@@ -772,21 +772,24 @@ BT_DEV double ref_hypot(double x, double y)
 
 // The comparison hypot(x, y) < r decided from s = x*x + y*y without the
 // square root: -1 when hypot(x, y) < r for certain, +1 when > r, 0 when s
-// lies within 1e-13 of r*r (or outside the range where that bound holds:
-// NaN, inf, r outside {0} and [1e-140, 1e140], s >= 1e280), where only
-// libm's value decides.  s carries at most ~2^-51 relative error and libm's hypot
-// is within 1 ulp of the true value, far inside the 1e-13 margin, so a
-// decided comparison agrees with libm's.
+// lies within 2^-43 (~1.1e-13) of r*r (or outside the range where that
+// bound holds: NaN, inf, r outside {0} and (2^-465, 2^465), s >= 2^930),
+// where only libm's value decides.  s carries at most ~2^-51 relative error
+// and libm's hypot is within 1 ulp of the true value, far inside the 2^-43
+// margin, so a decided comparison agrees with libm's.  The bounds are powers
+// of two so that each compare takes its constant as an instruction literal
+// instead of holding it in registers across the interface loop.
 BT_DEV int cmp_hypot(double x, double y, double r)
 {
     const double s = x*x + y*y;
-    if (r == 0.0) return (s > 0.0 && s < 1e280) ? 1 : 0;   // an annulus from 0
-    if (!(r > 1e-140 && r < 1e140)) return 0;
-    if (!(s < 1e280)) return 0;
+    if (r == 0.0) return (s > 0.0 && s < 0x1p930) ? 1 : 0;   // an annulus from 0
+    if (!(r > 0x1p-465 && r < 0x1p465)) return 0;
+    if (!(s < 0x1p930)) return 0;
     const double r2 = r*r;
     const double d = s - r2;
-    if (d > 1e-13*r2) return 1;
-    if (d < -1e-13*r2) return -1;
+    const double m = r2*0x1p-43;
+    if (d > m) return 1;
+    if (d < -m) return -1;
     return 0;
 }
 

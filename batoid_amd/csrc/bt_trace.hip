@@ -232,8 +232,8 @@ __device__ __forceinline__ void mlp_features(const bt_step* __restrict__ st,
 }
 
 // Waves per SIMD: the analytic-kinds kernels of modes 0 and 1 are held to
-// five (95 VGPRs; one 8-byte value per ray goes to scratch), which the
-// compiler would otherwise leave at four (97 VGPRs); the all-kinds kernel
+// five (93 VGPRs, no spills), so that a few more registers in a later change
+// cannot silently drop them to four; the all-kinds kernel
 // of mode 0 is held to four (some registers spill to scratch; the extra
 // wave hides more FP64 latency: 8 % faster on the TMA, 9-12 % on the
 // Newton-heavy optics), mode 2 to three (left alone the compiler would

@@ -33,9 +33,8 @@ BYTES_PER_RAY = 8 * 8 + 2 + 7 * 8 + 2
 # HBM bytes per ray of the same kernel on this workload, measured with
 # rocprofv3 FETCH_SIZE and WRITE_SIZE (each scaled by its reading for a
 # 1 GiB device copy: x2.000 and x1.000), from
-# profiles/trace_plan_tma_1e7_counters.md: the 124 algorithmic bytes plus
-# the one 8-byte register spill per ray of the five-wave build.
-MEASURED_TRAFFIC_PER_RAY = 132.03
+# profiles/trace_plan_tma_1e7_counters.md.
+MEASURED_TRAFFIC_PER_RAY = 124.05
 # FP64 flops per ray of the same kernel on this workload: 64 lanes x
 # (ADD + MUL + 2 FMA) FP64 wave instructions (SQ_INSTS_VALU_*_F64), same file
 # (current build's figure).
