@@ -128,6 +128,8 @@ SIGNATURES = {
     "bt_abi_version": (_I, []),
     "bt_last_error": (ctypes.c_char_p, []),
     "bt_struct_sizes": (_I, [ctypes.POINTER(ctypes.c_int64), _I]),
+    "bt_struct_field_count": (_I, [_I]),
+    "bt_struct_field_offset": (_L, [_I, ctypes.c_char_p]),
     "bt_device_info": (_I, [ctypes.POINTER(_I)] * 3),
     "bt_apply_forward_transform": (_I, [_V, _V, BtRays, _V]),
     "bt_apply_reverse_transform": (_I, [_V, _V, BtRays, _V]),
@@ -203,8 +205,27 @@ def load(path=LIB_PATH):
                 f"ABI mismatch: sizeof({st.__name__}) is {sz} in the library, "
                 f"{ctypes.sizeof(st)} in the binding"
             )
+    for which, st in enumerate(STRUCTS):
+        mismatch = layout_mismatch(lib, which, st)
+        if mismatch:
+            raise BatoidAMDError(f"ABI mismatch in {st.__name__}: {mismatch}")
     _lib = lib
     return lib
+
+
+def layout_mismatch(lib, which, st):
+    """'' if every field of ctypes struct ``st`` sits, by name, at the offset
+    the library reports for struct number ``which``, else a description."""
+    names = [f[0] for f in st._fields_]
+    count = lib.bt_struct_field_count(which)
+    if count != len(names):
+        return f"{count} fields in the library, {len(names)} in the binding"
+    bad = []
+    for n in names:
+        off = lib.bt_struct_field_offset(which, n.encode())
+        if off != getattr(st, n).offset:
+            bad.append(f"{n} at {off} (binding {getattr(st, n).offset})")
+    return "; ".join(bad)
 
 
 def check(rc, what=""):

@@ -226,10 +226,32 @@ def zernike(optic, theta_x, theta_y, wavelength, projection='postel', nx=32,
         mask = srays.torch("vignetted")
     G = zernike_gram(epRays.torch("x"), epRays.torch("y"), srays.torch("t"), t0, wavelength,
                      optic.pupilSize / 2, optic.pupilSize / 2 * eps, jmax, mask=mask)
+    return solve_normal_equations(G, jmax)
+
+
+def solve_normal_equations(G, jmax):
+    """Least-squares Zernike coefficients from the Gram matrix G of
+    [Z_1 .. Z_jmax, w] (zernike_gram), with the reference's rank check
+    (analysis.py:566-570: RuntimeError when the basis has rank < jmax).
+
+    The basis B (rows = unmasked rays) is not formed; its rank is judged on
+    A = B^T B.  Z_1 = 1, so A[0, 0] is the row count N: N < jmax is rank
+    deficient outright.  Otherwise an eigenvalue of A counts when it exceeds
+    lambda_max * max(N, jmax + 1) * eps -- a singular value of B above
+    s_max * sqrt(max(N, jmax + 1) * eps), since A's eigenvalues carry absolute
+    errors of order eps * lambda_max and smaller singular values cannot be
+    resolved from A.  (numpy's lstsq on B resolves down to
+    s_max * max(N, jmax + 1) * eps: a basis whose smallest singular value lies
+    between the two, condition number above ~1e6, is accepted by the
+    reference and rejected here.)"""
     A, b = G[:jmax, :jmax], G[:jmax, jmax]
     nrows = int(round(G[0, 0]))            # Z_1 = 1: the count of unmasked rays
-    s = np.sqrt(np.clip(np.linalg.eigvalsh(A), 0.0, None))
-    rank = int((s > s.max() * max(nrows, jmax + 1) * np.finfo(float).eps).sum())
+    if nrows < jmax:
+        raise RuntimeError(
+            f"Rank {nrows} matrix found when fitting {jmax} Zernike coefficients.  ")
+    lam = np.linalg.eigvalsh(A)
+    tol = lam.max() * max(nrows, jmax + 1) * np.finfo(float).eps
+    rank = int((lam > tol).sum()) if lam.max() > 0 else 0
     if rank < jmax:
         raise RuntimeError(
             f"Rank {rank} matrix found when fitting {jmax} Zernike coefficients.  ")

@@ -16,6 +16,8 @@
 #include <hip/hip_runtime.h>
 #include <string>
 #include <cstdio>
+#include <cstddef>
+#include <cstring>
 
 #include "bt_device.h"
 #include "bt_common.h"
@@ -400,6 +402,71 @@ int bt_struct_sizes(int64_t* sizes, int n)
     if (!sizes) return fail(BT_E_NULL, "bt_struct_sizes: null argument");
     for (int i = 0; i < n && i < 9; i++) sizes[i] = all[i];
     return 0;
+}
+
+namespace {
+struct FieldOffset { const char* name; int64_t offset; };
+#define BT_F(S, f) {#f, (int64_t)offsetof(S, f)}
+const FieldOffset kRaysFields[] = {
+    BT_F(bt_rays, x), BT_F(bt_rays, y), BT_F(bt_rays, z), BT_F(bt_rays, vx),
+    BT_F(bt_rays, vy), BT_F(bt_rays, vz), BT_F(bt_rays, t), BT_F(bt_rays, wavelength),
+    BT_F(bt_rays, flux), BT_F(bt_rays, vignetted), BT_F(bt_rays, failed), BT_F(bt_rays, n)};
+const FieldOffset kSurfaceFields[] = {
+    BT_F(bt_surface, kind), BT_F(bt_surface, nchild), BT_F(bt_surface, n1),
+    BT_F(bt_surface, n2), BT_F(bt_surface, flags), BT_F(bt_surface, pad_),
+    BT_F(bt_surface, p), BT_F(bt_surface, a0), BT_F(bt_surface, a1), BT_F(bt_surface, a2),
+    BT_F(bt_surface, a3)};
+const FieldOffset kMediumFields[] = {
+    BT_F(bt_medium, kind), BT_F(bt_medium, n), BT_F(bt_medium, p), BT_F(bt_medium, a0),
+    BT_F(bt_medium, a1)};
+const FieldOffset kObscFields[] = {
+    BT_F(bt_obsc, op), BT_F(bt_obsc, n), BT_F(bt_obsc, p), BT_F(bt_obsc, a0),
+    BT_F(bt_obsc, a1)};
+const FieldOffset kStepFields[] = {
+    BT_F(bt_step, action), BT_F(bt_step, surface), BT_F(bt_step, medium),
+    BT_F(bt_step, obsc), BT_F(bt_step, obsc_len), BT_F(bt_step, start), BT_F(bt_step, dr),
+    BT_F(bt_step, drot), BT_F(bt_step, mlp)};
+const FieldOffset kPlanFields[] = {
+    BT_F(bt_plan, nstep), BT_F(bt_plan, flags), BT_F(bt_plan, nmedia), BT_F(bt_plan, pad_),
+    BT_F(bt_plan, steps), BT_F(bt_plan, surfaces), BT_F(bt_plan, media),
+    BT_F(bt_plan, obscs)};
+const FieldOffset kSegmentFields[] = {
+    BT_F(bt_psf_segment, begin), BT_F(bt_psf_segment, end), BT_F(bt_psf_segment, chief),
+    BT_F(bt_psf_segment, time_ray), BT_F(bt_psf_segment, a1), BT_F(bt_psf_segment, a2)};
+const FieldOffset kRaygenFields[] = {
+    BT_F(bt_raygen, pattern), BT_F(bt_raygen, nring), BT_F(bt_raygen, nx),
+    BT_F(bt_raygen, ny), BT_F(bt_raygen, fx), BT_F(bt_raygen, fy), BT_F(bt_raygen, lx),
+    BT_F(bt_raygen, ly), BT_F(bt_raygen, lo), BT_F(bt_raygen, span),
+    BT_F(bt_raygen, pcg_state), BT_F(bt_raygen, pcg_inc), BT_F(bt_raygen, ring_off),
+    BT_F(bt_raygen, ring_rho), BT_F(bt_raygen, ring_dth), BT_F(bt_raygen, px),
+    BT_F(bt_raygen, py), BT_F(bt_raygen, flux), BT_F(bt_raygen, stop),
+    BT_F(bt_raygen, stop_dr), BT_F(bt_raygen, stop_drot), BT_F(bt_raygen, origin),
+    BT_F(bt_raygen, rot), BT_F(bt_raygen, vv), BT_F(bt_raygen, wavelength),
+    BT_F(bt_raygen, flux0)};
+const FieldOffset kCoatingFields[] = {
+    BT_F(bt_coating, reflectivity), BT_F(bt_coating, transmissivity)};
+#undef BT_F
+#define BT_TABLE(t) {t, (int)(sizeof(t) / sizeof(t[0]))}
+const struct { const FieldOffset* fields; int count; } kStructFields[9] = {
+    BT_TABLE(kRaysFields), BT_TABLE(kSurfaceFields), BT_TABLE(kMediumFields),
+    BT_TABLE(kObscFields), BT_TABLE(kStepFields), BT_TABLE(kPlanFields),
+    BT_TABLE(kSegmentFields), BT_TABLE(kRaygenFields), BT_TABLE(kCoatingFields)};
+#undef BT_TABLE
+}  // namespace
+
+int bt_struct_field_count(int which)
+{
+    if (which < 0 || which >= 9) return fail(BT_E_SIZE, "bt_struct_field_count: no such struct");
+    return kStructFields[which].count;
+}
+
+int64_t bt_struct_field_offset(int which, const char* field)
+{
+    if (which < 0 || which >= 9 || !field) return -1;
+    for (int i = 0; i < kStructFields[which].count; i++)
+        if (strcmp(kStructFields[which].fields[i].name, field) == 0)
+            return kStructFields[which].fields[i].offset;
+    return -1;
 }
 
 int bt_device_info(int* cu_count, int* arch_major, int* arch_minor)

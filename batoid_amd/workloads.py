@@ -4,9 +4,10 @@
   rays arrive travelling -z; 2 m aperture, 0.4 m central obscuration) and a
   flat Detector at its focus z = R/2, in vacuum.
 * ``tma()``: a 3-mirror anastigmat with three fused-silica lenses (Sellmeier),
-  a filter and a detector -- the LSST r-band prescription
-  (batoid/data/LSST/LSST_r.yaml:17-188) with the primary's even asphere
-  extended to five terms.  12 interfaces.
+  a flat filter (a Plane/Plane silica plate) and a detector -- the LSST
+  r-band prescription (batoid/data/LSST/LSST_r.yaml:17-188) with the
+  primary's even asphere extended to five terms and the filter's two
+  shallow spheres (R = 5.632 / 5.606 m there) made flat.  12 interfaces.
 * ``tma_rays()``: the hexapolar pupil of the TMA: asPolar(outer=4.18,
   inner=2.558, nrad=1120, naz=11080) = 9,998,544 rays.
 * ``tma_fields()``: 189 field points uniform in a 1.75 degree disk.
@@ -63,8 +64,8 @@ def tma_config(m1_coefs=(0.0, -1.381e-9, -4.3e-20, 2.1e-25, -1.5e-30)):
                  _lens('L2', 0.49487202, {'type': 'Plane'},
                        {'type': 'Asphere', 'R': 2.529, 'conic': -1.57,
                         'coefs': [0.0, -0.001656]}, 0.551, 0.03),
-                 _lens('Filter', 0.87445202, {'type': 'Sphere', 'R': 5.632},
-                       {'type': 'Sphere', 'R': 5.606}, 0.375, 0.0179),
+                 _lens('Filter', 0.87445202, {'type': 'Plane'}, {'type': 'Plane'},
+                       0.375, 0.0179),
                  _lens('L3', 0.94345202, {'type': 'Quadric', 'R': 3.169, 'conic': -0.962},
                        {'type': 'Sphere', 'R': -13.36}, 0.361, 0.06),
                  {'type': 'Detector', 'name': 'Detector', 'surface': {'type': 'Plane'},

@@ -209,6 +209,14 @@ const char* bt_last_error(void);
  * bt_psf_segment, bt_raygen, bt_coating, in that order, written to sizes[0..n-1]; lets a binding
  * check its layout. */
 int bt_struct_sizes(int64_t* sizes, int n);
+/* Field layout of struct number `which` (the order of bt_struct_sizes):
+ * bt_struct_field_count gives its number of fields (BT_E_SIZE for an
+ * unknown struct), bt_struct_field_offset the byte offset of the field
+ * named `field` as spelled in this header (-1 if it has no such field).
+ * With bt_struct_sizes a binding can check each of its fields by name, so
+ * two same-size fields declared in the wrong order are caught. */
+int bt_struct_field_count(int which);
+int64_t bt_struct_field_offset(int which, const char* field);
 /* Device properties the host uses for launch sizing. */
 int bt_device_info(int* cu_count, int* arch_major, int* arch_minor);
 

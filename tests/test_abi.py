@@ -43,3 +43,26 @@ def test_struct_layouts_match():
     assert lib.bt_struct_sizes(sizes, len(_lib.STRUCTS)) == 0
     assert list(sizes) == [ctypes.sizeof(s) for s in _lib.STRUCTS]
     assert lib.bt_abi_version() == 5
+
+
+def test_struct_field_offsets_match():
+    """Every field of every struct sits, by name, where the library puts it
+    (two same-size fields swapped in the binding would pass the size check)."""
+    lib = _lib.load()
+    for which, st in enumerate(_lib.STRUCTS):
+        assert lib.bt_struct_field_count(which) == len(st._fields_), st.__name__
+        assert _lib.layout_mismatch(lib, which, st) == "", st.__name__
+    assert lib.bt_struct_field_count(len(_lib.STRUCTS)) < 0
+    assert lib.bt_struct_field_offset(0, b"no_such_field") == -1
+
+
+def test_offset_check_catches_swapped_fields():
+    """A binding with two 8-byte fields swapped is rejected, field by field."""
+    lib = _lib.load()
+
+    class Reordered(ctypes.Structure):
+        _fields_ = [("transmissivity", ctypes.c_double), ("reflectivity", ctypes.c_double)]
+    assert ctypes.sizeof(Reordered) == ctypes.sizeof(_lib.BtCoating)
+    msg = _lib.layout_mismatch(lib, _lib.STRUCTS.index(_lib.BtCoating), Reordered)
+    assert "transmissivity" in msg and "reflectivity" in msg
+

@@ -154,6 +154,19 @@ def test_rank_deficient():
         bt.zernike(telescope, 0.0, 0.0, 625e-9, jmax=28, reference='chief')
 
 
+def test_fewer_rays_than_coefficients():
+    """0 < unmasked rays < jmax: rank deficient, RuntimeError (the reference's
+    lstsq rank check, analysis.py:566-570)."""
+    telescope = bt.Optic.fromYaml(os.path.join(GOLD, "optics", "LSST_r.yaml"))
+    rays = bt.RayVector.asGrid(optic=telescope, wavelength=625e-9, nx=6,
+                               dirCos=bt.utils.fieldToDirCos(0.0, 0.0))
+    telescope.trace(rays)
+    nvalid = int((~rays.vignetted).sum())
+    assert 0 < nvalid < 37
+    with pytest.raises(RuntimeError):
+        bt.zernike(telescope, 0.0, 0.0, 625e-9, nx=6, jmax=37)
+
+
 def test_rotated_optic_zernikes():
     """tests/test_Optic.py:181-247: a sub-item rotated and rotated back (or
     by the identity) leaves zernikeGQ unchanged to 1e-7 waves."""

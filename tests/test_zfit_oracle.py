@@ -5,6 +5,7 @@ uses, and the reference's Zemax HSC golden data -- Zernike coefficients
 import os
 
 import numpy as np
+import pytest
 
 import batoid_amd as bt
 from batoid_amd.analysis import zernike_table
@@ -90,3 +91,63 @@ def test_oracle_wavefront_vs_zemax_hsc():
     # as the reference, the fit uses batoid's values under Zemax's mask,
     # vignetted or not (a masked array's data)
     np.testing.assert_allclose(zc[4:], bc[4:], rtol=0, atol=0.01)
+
+
+def _gram(x, y, w, jmax):
+    """[Z_1 .. Z_jmax, w] Gram matrix in numpy (the device pass's output)."""
+    C, nmax = zernike_table(jmax)
+    cols = []
+    for j in range(jmax):
+        v = np.zeros_like(x)
+        for d in range(nmax + 1):
+            for q in range(d + 1):
+                c = C[j, d * (d + 1) // 2 + q]
+                if c:
+                    v += c * x ** (d - q) * y ** q
+        cols.append(v)
+    B = np.stack(cols + [w], axis=1)
+    return B.T @ B, B[:, :jmax]
+
+
+def _reference_rank(B, w):
+    """numpy lstsq's rank with the reference's rcond (analysis.py:566)."""
+    return np.linalg.lstsq(B, w, rcond=None)[2]
+
+
+def test_normal_equations_rank_rule():
+    """The host solve of zernike(): rank-deficient fits raise RuntimeError as
+    the reference's lstsq check does (analysis.py:566-570), full-rank fits
+    equal lstsq's solution."""
+    from batoid_amd.analysis import solve_normal_equations
+    rng = np.random.default_rng(3)
+    jmax = 22
+    # fewer rows than coefficients
+    for n in (1, 5, 21):
+        r = np.sqrt(rng.uniform(0, 1, n))
+        th = rng.uniform(0, 2 * np.pi, n)
+        x, y = r * np.cos(th), r * np.sin(th)
+        G, B = _gram(x, y, rng.normal(size=n), jmax)
+        assert _reference_rank(B, x) < jmax
+        with pytest.raises(RuntimeError):
+            solve_normal_equations(G, jmax)
+    # many rows on a line or on a circle: degenerate bases
+    t = rng.uniform(-1, 1, 500)
+    th = rng.uniform(0, 2 * np.pi, 500)
+    for x, y in ((t, np.zeros_like(t)), (0.7 * np.cos(th), 0.7 * np.sin(th))):
+        G, B = _gram(x, y, rng.normal(size=len(x)), jmax)
+        assert _reference_rank(B, x) < jmax
+        with pytest.raises(RuntimeError):
+            solve_normal_equations(G, jmax)
+    # a full disk: full rank, the lstsq solution
+    r = np.sqrt(rng.uniform(0, 1, 2000))
+    th = rng.uniform(0, 2 * np.pi, 2000)
+    x, y = r * np.cos(th), r * np.sin(th)
+    truth = rng.normal(size=jmax)
+    _, B = _gram(x, y, np.zeros_like(x), jmax)
+    w = B @ truth + 1e-3 * rng.normal(size=len(x))
+    G, B = _gram(x, y, w, jmax)
+    assert _reference_rank(B, w) == jmax
+    got = solve_normal_equations(G, jmax)
+    assert got[0] == 0.0
+    np.testing.assert_allclose(got[1:], np.linalg.lstsq(B, w, rcond=None)[0],
+                               rtol=0, atol=1e-9)
