@@ -257,8 +257,10 @@ def test_default_lattice_psf_is_the_obscured_airy(method):
     assert psf.array.shape == (2 * nx, 2 * nx)          # pad factor 2
     du = diam / (nx - 2)
     scale = lam * focal / (2 * nx * du)
-    np.testing.assert_allclose(np.abs(psf.primitiveVectors), np.diag([scale, scale]),
-                               rtol=2e-3, atol=1e-13)
+    prim = psf.primitiveVectors
+    np.testing.assert_allclose(np.abs(np.diag(prim)), [scale, scale], rtol=2e-3)
+    # off-diagonal terms come only from the fit's sampling (~1e-5 relative)
+    assert np.abs(prim[0, 1]) < 1e-4 * scale and np.abs(prim[1, 0]) < 1e-4 * scale
     I = psf.array / psf.array.max()
     xy = psf.coords                                     # [i, j] -> (x, y); image is [j, i]
     r = np.hypot(xy[..., 0].T, xy[..., 1].T)
