@@ -169,6 +169,8 @@ SIGNATURES = {
     "bt_huygens_workspace_bytes": (_L, [_L, _I, _I]),
     "bt_huygens_direct": (_I, [BtRays, _V, _V, _V, _I, _I, _D, _I, _V, _V, _L, _V]),
     "bt_huygens_gemm": (_I, [BtRays, _V, _V, _V, _I, _I, _D, _I, _I, _V, _V, _L, _V]),
+    "bt_huygens_noalloc_scratch_bytes": (_L, []),
+    "bt_huygens_gemm_noalloc": (_I, [BtRays, _V, _V, _V, _I, _I, _D, _I, _I, _V, _V]),
     "bt_huygens_batch_workspace_bytes": (_L, [_L, _I, _I, _I, _I]),
     "bt_huygens_batch": (_I, [BtRays, _V, _I, _I, _I, _I, _I, _I, _I, _V, _V, _L, _V]),
     "bt_zernike_workspace_bytes": (_L, [_I]),

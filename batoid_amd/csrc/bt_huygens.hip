@@ -36,6 +36,7 @@
 // derives the other blocks by rotations precomputed per ray in FP64.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <mutex>
 #include <string>
 
 #include "bt_device.h"
@@ -791,19 +792,118 @@ extern "C" int64_t bt_huygens_workspace_bytes(int64_t n, int nx, int ny)
     return d > g ? d : g;
 }
 
+namespace bt {
+
+int gemm_args_ok(const char* name, const bt_rays& rays, const double* p0, const double* a1,
+                 const double* a2, int nx, int ny, int precision, const double* amp_out)
+{
+    if (!rays_ok(rays) || !p0 || !a1 || !a2 || !amp_out)
+        return fail(BT_E_NULL, std::string(name) + ": null argument");
+    if (nx <= 0 || ny <= 0 || nx > 8192 || ny > 8192)
+        return fail(BT_E_SIZE, std::string(name) + ": image size must be 1..8192 per side");
+    if (rays.n > (int64_t)1 << 30)
+        return fail(BT_E_SIZE, std::string(name) + ": more than 2^30 rays");
+    if (precision != BT_PSF_FP32 && precision != BT_PSF_BF16X3)
+        return fail(BT_E_KIND, std::string(name) + ": unknown precision");
+    return 0;
+}
+
+// The launches of bt_huygens_gemm on a workspace of workspace_bytes bytes
+// (arguments already checked).
+int huygens_gemm_launch(bt_rays rays, const double* p0, const double* a1, const double* a2,
+                        int nx, int ny, double t, int ignore_vignetted, int precision,
+                        double* amp_out, void* workspace, int64_t workspace_bytes,
+                        void* stream);
+
+// ---- allocation-free variant --------------------------------------------
+// bt_huygens_gemm_noalloc takes its scratch (the compacted per-ray
+// parameters, per-block counts and offsets, and the K-split partial images
+// of gemm_layout) from this fixed device array instead of a caller
+// workspace.  Each GPU the library runs on holds its own copy until the
+// process ends; calls on one GPU share it.
+constexpr int64_t kNoAllocScratchBytes = (int64_t)128 << 20;
+constexpr int kMaxDevices = 64;
+
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+}  // namespace bt
+
+static __device__ __align__(256) unsigned char scratch[bt::kNoAllocScratchBytes];
+
+namespace bt {
+
+// Device address of the current device's `scratch`, looked up on the first
+// call on each device (with that device current) and kept: the host-side
+// name `scratch` is a stand-in, never a device address.
+int noalloc_scratch(void** out)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return fail((int)e, "bt_huygens_gemm_noalloc: hipGetDevice failed");
+    if (dev < 0 || dev >= kMaxDevices)
+        return fail(BT_E_SIZE, "bt_huygens_gemm_noalloc: device index beyond 63");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(e != hipSuccess ? (int)e : BT_E_NULL,
+                        std::string("bt_huygens_gemm_noalloc: hipGetSymbolAddress(scratch) "
+                                    "failed: ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return 0;
+}
+
+}  // namespace bt
+
+extern "C" int64_t bt_huygens_noalloc_scratch_bytes(void)
+{
+    return bt::kNoAllocScratchBytes;
+}
+
+extern "C" int bt_huygens_gemm_noalloc(bt_rays rays, const double* p0, const double* a1,
+                                       const double* a2, int nx, int ny, double t,
+                                       int ignore_vignetted, int precision, double* amp_out,
+                                       void* stream)
+{
+    int rc = gemm_args_ok("bt_huygens_gemm_noalloc", rays, p0, a1, a2, nx, ny, precision,
+                          amp_out);
+    if (rc) return rc;
+    const int64_t need = gemm_layout(rays.n, nx, ny).bytes;
+    if (need > kNoAllocScratchBytes)
+        return fail(BT_E_SIZE,
+                    "bt_huygens_gemm_noalloc: " + std::to_string(rays.n) + " rays and a " +
+                    std::to_string(nx) + " x " + std::to_string(ny) + " image need " +
+                    std::to_string(need) + " bytes of scratch, beyond the fixed " +
+                    std::to_string(kNoAllocScratchBytes) +
+                    " (use bt_huygens_gemm with a workspace)");
+    void* ws = nullptr;
+    rc = noalloc_scratch(&ws);
+    if (rc) return rc;
+    return huygens_gemm_launch(rays, p0, a1, a2, nx, ny, t, ignore_vignetted, precision,
+                               amp_out, ws, kNoAllocScratchBytes, stream);
+}
+
 extern "C" int bt_huygens_gemm(bt_rays rays, const double* p0, const double* a1,
                                const double* a2, int nx, int ny, double t,
                                int ignore_vignetted, int precision, double* amp_out,
                                void* workspace, int64_t workspace_bytes, void* stream)
 {
-    if (!rays_ok(rays) || !p0 || !a1 || !a2 || !amp_out || !workspace)
-        return fail(BT_E_NULL, "bt_huygens_gemm: null argument");
-    if (nx <= 0 || ny <= 0 || nx > 8192 || ny > 8192)
-        return fail(BT_E_SIZE, "bt_huygens_gemm: image size must be 1..8192 per side");
-    if (rays.n > (int64_t)1 << 30)
-        return fail(BT_E_SIZE, "bt_huygens_gemm: more than 2^30 rays");
-    if (precision != BT_PSF_FP32 && precision != BT_PSF_BF16X3)
-        return fail(BT_E_KIND, "bt_huygens_gemm: unknown precision");
+    int rc = gemm_args_ok("bt_huygens_gemm", rays, p0, a1, a2, nx, ny, precision, amp_out);
+    if (rc) return rc;
+    if (!workspace) return fail(BT_E_NULL, "bt_huygens_gemm: null argument");
+    return huygens_gemm_launch(rays, p0, a1, a2, nx, ny, t, ignore_vignetted, precision,
+                               amp_out, workspace, workspace_bytes, stream);
+}
+
+int bt::huygens_gemm_launch(bt_rays rays, const double* p0, const double* a1,
+                            const double* a2, int nx, int ny, double t, int ignore_vignetted,
+                            int precision, double* amp_out, void* workspace,
+                            int64_t workspace_bytes, void* stream)
+{
     const GemmLayout L = gemm_layout(rays.n, nx, ny);
     if (workspace_bytes < L.bytes) return fail(BT_E_SIZE, "bt_huygens_gemm: workspace too small");
     char* ws = static_cast<char*>(workspace);

@@ -471,6 +471,25 @@ int bt_huygens_gemm(bt_rays rays, const double* p0, const double* a1,
                     int ignore_vignetted, int precision, double* amp_out,
                     void* workspace, int64_t workspace_bytes, void* stream);
 
+/* Allocation-free bt_huygens_gemm -- no counterpart in the reference; for
+ * callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, kernels and results (bit for bit) as bt_huygens_gemm, but the
+ * scratch comes from a fixed device array inside the library
+ * (bt_huygens_noalloc_scratch_bytes() = 128 MiB; each GPU the library runs
+ * on holds its own copy until the process ends) instead of a caller
+ * workspace.  A request whose scratch (bt_huygens_workspace_bytes(n, nx, ny))
+ * exceeds it returns BT_E_SIZE before anything is launched.  Every call on
+ * one GPU uses that GPU's array: calls that overlap on one GPU (two host
+ * threads, or two streams whose work may run concurrently) race and must
+ * be serialised by the caller.  Makes no hipMalloc, hipFree or memset call;
+ * the first call on each device looks up the array's device address.    */
+int64_t bt_huygens_noalloc_scratch_bytes(void);
+int bt_huygens_gemm_noalloc(bt_rays rays, const double* p0, const double* a1,
+                            const double* a2, int nx, int ny, double t,
+                            int ignore_vignetted, int precision, double* amp_out,
+                            void* stream);
+
 /* ---- batched Huygens PSFs ---------------------------------------------------
  * One nx x ny PSF per segment of a ray batch, all in one call (the nearest
  * reference is a Python loop of huygensPSF calls, analysis.py:181-305, over

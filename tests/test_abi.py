@@ -66,3 +66,23 @@ def test_offset_check_catches_swapped_fields():
     msg = _lib.layout_mismatch(lib, _lib.STRUCTS.index(_lib.BtCoating), Reordered)
     assert "transmissivity" in msg and "reflectivity" in msg
 
+
+def test_noalloc_gemm_refuses_oversized_requests_without_a_gpu():
+    """The allocation-free PSF entry point checks its fixed scratch on the
+    host: a request beyond it is refused with BT_E_SIZE before any HIP call
+    (the pointers below are never dereferenced)."""
+    lib = _lib.load()
+    cap = lib.bt_huygens_noalloc_scratch_bytes()
+    r = _lib.BtRays()
+    for k, _ in _lib.BtRays._fields_[:-1]:
+        setattr(r, k, 0x1000)
+    r.n = 1 << 22
+    d = (ctypes.c_double * 2)(0.0, 0.0)
+    rc = lib.bt_huygens_gemm_noalloc(r, d, d, d, 64, 64, 0.0, 1, _lib.PSF_FP32,
+                                     ctypes.c_void_p(0x1000), None)
+    assert rc == -2
+    msg = lib.bt_last_error().decode()
+    assert "scratch" in msg and str(cap) in msg
+    # and null pointers are refused first
+    assert lib.bt_huygens_gemm_noalloc(_lib.BtRays(), d, d, d, 64, 64, 0.0, 1,
+                                       _lib.PSF_FP32, None, None) == -1
