@@ -113,6 +113,7 @@ PLAN_NO_REFRACT = 1
 PLAN_FAST_START = 2
 PLAN_MLP_START = 4
 PLAN_ANALYTIC = 8
+PLAN_SUM_APB = 16
 ANALYTIC_KINDS = (PLANE, SPHERE, PARABOLOID, QUADRIC, ASPHERE, TILTED)
 PSF_FP32, PSF_BF16X3 = 0, 1
 GEN_POINTS, GEN_GRID, GEN_GRID_RANDOM, GEN_POLAR, GEN_POLAR_RANDOM = range(5)

@@ -214,15 +214,110 @@ BT_DEV double asphere_dzdr(const bt_surface* s, double r)
     return result;
 }
 
+BT_DEV double asphere_sag(const bt_surface* s, double x, double y)
+{                                                    // asphere.cpp:37-46
+    double r2 = x*x + y*y;
+    double rr = 1.0;
+    double result = quadric_sag(s, x, y);
+    const double* __restrict__ c = s->a0;
+    const int n = s->n1;
+    for (int i = 0; i < n; i++) {
+        rr *= r2;
+        result += c[i]*rr;
+    }
+    return result;
+}
+
+// Normal of a Quadric (kAsphere false) or an Asphere (true): quadric.cpp:27-39,
+// asphere.cpp:48-62.
+template <bool kAsphere>
+BT_DEV void conic_normal(const bt_surface* s, double x, double y,
+                         double& nx, double& ny, double& nz)
+{
+    double r = sqrt(x*x + y*y);
+    if (r == 0.0) {
+        nx = 0.0; ny = 0.0; nz = 1.0;
+    } else {
+        double dzdr = kAsphere ? asphere_dzdr(s, r) : quadric_dzdr(s, r);
+        nz = 1/sqrt(1+dzdr*dzdr);
+        nx = -x/r*dzdr*nz;
+        ny = -y/r*dzdr*nz;
+    }
+}
+
+BT_DEV double poly_sag(const bt_surface* s, double x, double y)
+{                                                    // polynomialSurface.cpp:35-37
+    return horner2d(x-s->p[0], y-s->p[1], s->a0, s->n1, s->n2);
+}
+
+BT_DEV void poly_normal(const bt_surface* s, double x, double y,
+                        double& nx, double& ny, double& nz)
+{                                                    // polynomialSurface.cpp:39-48
+    double gx = -horner2d(x-s->p[0], y-s->p[1], s->a1, s->n1-1, s->n2-1);
+    double gy = -horner2d(x-s->p[0], y-s->p[1], s->a2, s->n1-1, s->n2-1);
+    nz = 1./sqrt(gx*gx + gy*gy + 1);
+    nx = gx*nz;
+    ny = gy*nz;
+}
+
+BT_DEV void table_normal(const bt_surface* s, double x, double y,
+                         double& nx, double& ny, double& nz)
+{                                                    // bicubic.cpp:23-40
+    double dxdz, dydz;
+    table_grad(s, x, y, dxdz, dydz);
+    if (__builtin_isnan(dxdz)) {
+        nx = __builtin_nan(""); ny = nx; nz = nx;
+        return;
+    }
+    nz = 1/sqrt(1 + dxdz*dxdz + dydz*dydz);
+    nx = -dxdz*nz;
+    ny = -dydz*nz;
+}
+
+// Bicubic sag and normal at one point, sharing the cell (bit-identical to
+// table_eval and table_normal).
+BT_DEV void table_sag_normal(const bt_surface* s, double x, double y, double& sag,
+                             double& nx, double& ny, double& nz)
+{                                                    // bicubic.cpp:19-40
+    double dxdz, dydz;
+    table_eval_grad(s, x, y, sag, dxdz, dydz);
+    if (__builtin_isnan(dxdz)) {
+        nx = __builtin_nan(""); ny = nx; nz = nx;
+        return;
+    }
+    nz = 1/sqrt(1 + dxdz*dxdz + dydz*dydz);
+    nx = -dxdz*nz;
+    ny = -dydz*nz;
+}
+
 // ---------------------------------------------------------------------------
-// Surface kinds a kernel is compiled for.  kKindsAnalytic covers Plane,
-// Sphere, Paraboloid, Quadric, Asphere and Tilted surfaces without Sums: the
-// Bicubic, polynomial and Sum code paths are left out, which lets a plan
-// kernel for such optics keep far fewer registers.  Under it, any other kind
-// evaluates to NaN (and so fails).
+// Surface kinds a kernel is compiled for.
+//  * kKindsAll: every kind, Sums of any children (a switch over the kind of
+//    each child inside the Sum's loop);
+//  * kKindsAnalytic: Plane, Sphere, Paraboloid, Quadric, Asphere and Tilted,
+//    no Sums -- the Bicubic, polynomial and Sum code paths are left out,
+//    which lets a plan kernel for such optics keep far fewer registers;
+//  * kKindsSumAPB: those analytic leaves plus Sums of exactly (Asphere,
+//    Polynomial, Bicubic), in that order (a perturbed conic mirror: figure,
+//    low-order Zernike error, measured high-frequency map).  The Sum's three
+//    children are evaluated by their own functions with no kind switch, so
+//    each Newton step carries only the registers those three need.
+// Under a restricted set any other kind (or Sum layout) evaluates to NaN and
+// so fails.
 // ---------------------------------------------------------------------------
 constexpr int kKindsAll = 0;
 constexpr int kKindsAnalytic = 1;
+constexpr int kKindsSumAPB = 2;
+
+// The kind set of leaves (records that are not Sums) under set K.
+constexpr int leaf_set(int K) { return K == kKindsAll ? kKindsAll : kKindsAnalytic; }
+
+// A Sum record laid out as kKindsSumAPB requires (records s+1..s+3 follow it).
+BT_DEV bool sum_is_apb(const bt_surface* s)
+{
+    return s->nchild == 3 && s[1].kind == BT_ASPHERE && s[2].kind == BT_POLYNOMIAL &&
+           s[3].kind == BT_BICUBIC;
+}
 
 // ---------------------------------------------------------------------------
 // Leaf surfaces: sag and normal
@@ -245,26 +340,15 @@ BT_DEV double leaf_sag(const bt_surface* s, double x, double y)
         return 0.0;
     case BT_QUADRIC:                                 // quadric.cpp:18-25
         return quadric_sag(s, x, y);
-    case BT_ASPHERE: {                               // asphere.cpp:37-46
-        double r2 = x*x + y*y;
-        double rr = 1.0;
-        double result = quadric_sag(s, x, y);
-        const double* __restrict__ c = s->a0;
-        const int n = s->n1;
-        for (int i = 0; i < n; i++) {
-            rr *= r2;
-            result += c[i]*rr;
-        }
-        return result;
-    }
+    case BT_ASPHERE:                                 // asphere.cpp:37-46
+        return asphere_sag(s, x, y);
     case BT_TILTED:                                  // tilted.cpp:16-18
         return x*s->p[0] + y*s->p[1];
     case BT_BICUBIC:                                 // bicubic.cpp:19-21
         if constexpr (K == kKindsAll) return table_eval(s, x, y);
         break;
     case BT_POLYNOMIAL:                              // polynomialSurface.cpp:35-37
-        if constexpr (K == kKindsAll)
-            return horner2d(x-s->p[0], y-s->p[1], s->a0, s->n1, s->n2);
+        if constexpr (K == kKindsAll) return poly_sag(s, x, y);
         break;
     }
     return __builtin_nan("");
@@ -325,61 +409,76 @@ BT_DEV void leaf_normal(const bt_surface* s, double x, double y,
         return;
     }
     case BT_BICUBIC:                                 // bicubic.cpp:23-40
-        if constexpr (K == kKindsAll) {
-            double dxdz, dydz;
-            table_grad(s, x, y, dxdz, dydz);
-            if (__builtin_isnan(dxdz)) {
-                nx = __builtin_nan(""); ny = nx; nz = nx;
-                return;
-            }
-            nz = 1/sqrt(1 + dxdz*dxdz + dydz*dydz);
-            nx = -dxdz*nz;
-            ny = -dydz*nz;
-            return;
-        }
+        if constexpr (K == kKindsAll) { table_normal(s, x, y, nx, ny, nz); return; }
         break;
     case BT_POLYNOMIAL:                              // polynomialSurface.cpp:39-48
-        if constexpr (K == kKindsAll) {
-            double gx = -horner2d(x-s->p[0], y-s->p[1], s->a1, s->n1-1, s->n2-1);
-            double gy = -horner2d(x-s->p[0], y-s->p[1], s->a2, s->n1-1, s->n2-1);
-            nz = 1./sqrt(gx*gx + gy*gy + 1);
-            nx = gx*nz;
-            ny = gy*nz;
-            return;
-        }
+        if constexpr (K == kKindsAll) { poly_normal(s, x, y, nx, ny, nz); return; }
         break;
     }
     nx = __builtin_nan(""); ny = nx; nz = nx;
 }
 
-// Sum records are followed by their children (sum.cpp:23-45).
+// Sum records are followed by their children (sum.cpp:23-45).  The Sum's
+// sag is 0 + child sags in child order, its normal from the sum of the
+// children's slopes (-n_x/n_z, -n_y/n_z) in child order, so the unrolled
+// kKindsSumAPB forms below are the same operations as the loops.
 template <int K = kKindsAll>
 BT_DEV double surf_sag(const bt_surface* s, double x, double y)
 {
-    if (K != kKindsAll || s->kind != BT_SUM) return leaf_sag<K>(s, x, y);
+    constexpr int L = leaf_set(K);
+    if (K == kKindsAnalytic || s->kind != BT_SUM) return leaf_sag<L>(s, x, y);
     double result = 0.0;
+    if constexpr (K == kKindsSumAPB) {
+        if (!sum_is_apb(s)) return __builtin_nan("");
+        result += asphere_sag(s + 1, x, y);
+        result += poly_sag(s + 2, x, y);
+        result += table_eval(s + 3, x, y);
+        return result;
+    }
     const int nc = s->nchild;
     for (int i = 0; i < nc; i++)
-        result += leaf_sag<K>(s + 1 + i, x, y);
+        result += leaf_sag<L>(s + 1 + i, x, y);
     return result;
+}
+
+BT_DEV void add_slope(double& sx, double& sy, double nx, double ny, double nz)
+{
+    sx += nx/nz;
+    sy += ny/nz;
+}
+
+BT_DEV void sum_normal_from_slopes(double sx, double sy, double& nx, double& ny, double& nz)
+{
+    nz = 1./sqrt(sx*sx + sy*sy + 1);
+    nx = sx*nz;
+    ny = sy*nz;
 }
 
 template <int K = kKindsAll>
 BT_DEV void surf_normal(const bt_surface* s, double x, double y,
                         double& nx, double& ny, double& nz)
 {
-    if (K != kKindsAll || s->kind != BT_SUM) { leaf_normal<K>(s, x, y, nx, ny, nz); return; }
+    constexpr int L = leaf_set(K);
+    if (K == kKindsAnalytic || s->kind != BT_SUM) { leaf_normal<L>(s, x, y, nx, ny, nz); return; }
     double sx = 0.0, sy = 0.0;
+    double tnx, tny, tnz;
+    if constexpr (K == kKindsSumAPB) {
+        if (!sum_is_apb(s)) { nx = __builtin_nan(""); ny = nx; nz = nx; return; }
+        conic_normal<true>(s + 1, x, y, tnx, tny, tnz);
+        add_slope(sx, sy, tnx, tny, tnz);
+        poly_normal(s + 2, x, y, tnx, tny, tnz);
+        add_slope(sx, sy, tnx, tny, tnz);
+        table_normal(s + 3, x, y, tnx, tny, tnz);
+        add_slope(sx, sy, tnx, tny, tnz);
+        sum_normal_from_slopes(sx, sy, nx, ny, nz);
+        return;
+    }
     const int nc = s->nchild;
     for (int i = 0; i < nc; i++) {
-        double tnx, tny, tnz;
-        leaf_normal<K>(s + 1 + i, x, y, tnx, tny, tnz);
-        sx += tnx/tnz;
-        sy += tny/tnz;
+        leaf_normal<L>(s + 1 + i, x, y, tnx, tny, tnz);
+        add_slope(sx, sy, tnx, tny, tnz);
     }
-    nz = 1./sqrt(sx*sx + sy*sy + 1);
-    nx = sx*nz;
-    ny = sy*nz;
+    sum_normal_from_slopes(sx, sy, nx, ny, nz);
 }
 
 // Sag and normal at one point (the reference evaluates them separately; the
@@ -388,16 +487,8 @@ template <int K = kKindsAll>
 BT_DEV void leaf_sag_normal(const bt_surface* s, double x, double y, double& sag,
                             double& nx, double& ny, double& nz)
 {
-    if (K == kKindsAll && s->kind == BT_BICUBIC) {                     // bicubic.cpp:19-40
-        double dxdz, dydz;
-        table_eval_grad(s, x, y, sag, dxdz, dydz);
-        if (__builtin_isnan(dxdz)) {
-            nx = __builtin_nan(""); ny = nx; nz = nx;
-            return;
-        }
-        nz = 1/sqrt(1 + dxdz*dxdz + dydz*dydz);
-        nx = -dxdz*nz;
-        ny = -dydz*nz;
+    if (K == kKindsAll && s->kind == BT_BICUBIC) {
+        table_sag_normal(s, x, y, sag, nx, ny, nz);
         return;
     }
     sag = leaf_sag<K>(s, x, y);
@@ -408,20 +499,36 @@ template <int K = kKindsAll>
 BT_DEV void surf_sag_normal(const bt_surface* s, double x, double y, double& sag,
                             double& nx, double& ny, double& nz)
 {
-    if (K != kKindsAll || s->kind != BT_SUM) { leaf_sag_normal<K>(s, x, y, sag, nx, ny, nz); return; }
+    constexpr int L = leaf_set(K);
+    if (K == kKindsAnalytic || s->kind != BT_SUM) {
+        leaf_sag_normal<L>(s, x, y, sag, nx, ny, nz);
+        return;
+    }
     double result = 0.0, sx = 0.0, sy = 0.0;
+    double ts, tnx, tny, tnz;
+    if constexpr (K == kKindsSumAPB) {
+        if (!sum_is_apb(s)) { sag = __builtin_nan(""); nx = sag; ny = sag; nz = sag; return; }
+        result += asphere_sag(s + 1, x, y);
+        conic_normal<true>(s + 1, x, y, tnx, tny, tnz);
+        add_slope(sx, sy, tnx, tny, tnz);
+        result += poly_sag(s + 2, x, y);
+        poly_normal(s + 2, x, y, tnx, tny, tnz);
+        add_slope(sx, sy, tnx, tny, tnz);
+        table_sag_normal(s + 3, x, y, ts, tnx, tny, tnz);
+        result += ts;
+        add_slope(sx, sy, tnx, tny, tnz);
+        sag = result;
+        sum_normal_from_slopes(sx, sy, nx, ny, nz);
+        return;
+    }
     const int nc = s->nchild;
     for (int i = 0; i < nc; i++) {
-        double ts, tnx, tny, tnz;
-        leaf_sag_normal<K>(s + 1 + i, x, y, ts, tnx, tny, tnz);
+        leaf_sag_normal<L>(s + 1 + i, x, y, ts, tnx, tny, tnz);
         result += ts;
-        sx += tnx/tnz;
-        sy += tny/tnz;
+        add_slope(sx, sy, tnx, tny, tnz);
     }
     sag = result;
-    nz = 1./sqrt(sx*sx + sy*sy + 1);
-    nx = sx*nz;
-    ny = sy*nz;
+    sum_normal_from_slopes(sx, sy, nx, ny, nz);
 }
 
 BT_DEV bool same_bits(double a, double b)
@@ -585,10 +692,12 @@ BT_DEV bool surf_tti(const bt_surface* s, double x, double y, double z,
                      double vx, double vy, double vz, double& dt, int niter,
                      NormalMemo* memo = nullptr)
 {
-    if (K != kKindsAll || s->kind != BT_SUM)
-        return leaf_tti<K>(s, x, y, z, vx, vy, vz, dt, niter, memo);
+    constexpr int L = leaf_set(K);
+    if (K == kKindsAnalytic || s->kind != BT_SUM)
+        return leaf_tti<L>(s, x, y, z, vx, vy, vz, dt, niter, memo);
+    if (K == kKindsSumAPB && !sum_is_apb(s)) return false;
     // sum.cpp:47-56: first child's intersection seeds Newton on the Sum
-    if (!leaf_tti<K>(s + 1, x, y, z, vx, vy, vz, dt, niter)) return false;
+    if (!leaf_tti<L>(s + 1, x, y, z, vx, vy, vz, dt, niter)) return false;
     return newton<K>(s, x, y, z, vx, vy, vz, dt, niter, memo);
 }
 
@@ -599,7 +708,7 @@ template <int K = kKindsAll>
 BT_DEV bool conic_start(const bt_surface* s, double x, double y, double z, double vx,
                         double vy, double vz, double& dt)
 {
-    const bt_surface* c = (K == kKindsAll && s->kind == BT_SUM) ? s + 1 : s;
+    const bt_surface* c = (K != kKindsAnalytic && s->kind == BT_SUM) ? s + 1 : s;
     dt = 0.0;
     switch (c->kind) {
     case BT_ASPHERE:
@@ -609,7 +718,7 @@ BT_DEV bool conic_start(const bt_surface* s, double x, double y, double z, doubl
     case BT_SPHERE:
     case BT_PARABOLOID:
     case BT_TILTED:
-        return leaf_tti<K>(c, x, y, z, vx, vy, vz, dt, 0);
+        return leaf_tti<leaf_set(K)>(c, x, y, z, vx, vy, vz, dt, 0);
     default:
         return true;
     }
@@ -623,20 +732,23 @@ BT_DEV bool newton_surface(const bt_surface* s)
 }
 
 constexpr double kFastTol = 1e-14;   // sag gap accepted by the fast starts [m]
-constexpr int kFastIter = 8;
 
-// Newton on the full surface from `dt` until |sag - z| <= kFastTol.  At
-// 1e-13 m the residual position errors, carried through a three-mirror
-// optic, reach ~1e-14 in the final direction cosines; 1e-14 m keeps them an
-// order of magnitude below that.
+// Newton on the full surface from `dt` until |sag - z| <= kFastTol, with at
+// most `max_iter` updates (the reference's niter: a fast start never takes
+// more iterations than the reference would, so a ray whose reference
+// iteration from the same start has not converged by then is handed back to
+// the reference path, which decides it).  At 1e-13 m the residual position
+// errors, carried through a three-mirror optic, reach ~1e-14 in the final
+// direction cosines; 1e-14 m keeps them an order of magnitude below that.
 template <int K = kKindsAll>
 BT_DEV bool newton_fast(const bt_surface* s, double x, double y, double z,
-                        double vx, double vy, double vz, double& dt, NormalMemo* memo)
+                        double vx, double vy, double vz, double& dt, NormalMemo* memo,
+                        int max_iter)
 {
     double rPx = x+vx*dt, rPy = y+vy*dt, rPz = z+vz*dt;
     double sz, nx, ny, nz;
     surf_sag_normal<K>(s, rPx, rPy, sz, nx, ny, nz);
-    for (int it = 0; it < kFastIter; it++) {
+    for (int it = 0; it < max_iter; it++) {
         if (fabs(sz-rPz) <= kFastTol) break;
         double d = (rPx-x)*nx + (rPy-y)*ny + (sz-z)*nz;
         dt = d / (nx*vx + ny*vy + nz*vz);
@@ -663,7 +775,7 @@ BT_DEV bool fast_tti(const bt_surface* s, double x, double y, double z, double v
         double d0;
         const bool ok = conic_start<K>(s, x, y, z, vx, vy, vz, d0);
         d0 += correction;
-        if (ok && d0 == d0 && newton_fast<K>(s, x, y, z, vx, vy, vz, d0, memo)) {
+        if (ok && d0 == d0 && newton_fast<K>(s, x, y, z, vx, vy, vz, d0, memo, niter)) {
             dt = d0;
             return true;
         }

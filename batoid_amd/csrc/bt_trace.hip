@@ -245,6 +245,7 @@ template <int Mode, int K>
 constexpr int plan_waves_per_simd()
 {
     if (K == kKindsAnalytic && Mode < 2) return 5;
+    if (K == kKindsSumAPB) return Mode == 2 ? 3 : 4;
     return Mode == 0 ? 4 : (Mode == 2 ? 3 : 1);
 }
 
@@ -596,18 +597,25 @@ int bt_trace_plan(bt_plan plan, bt_rays in, bt_rays out, int niter, void* stream
     if (niter < 0) return fail(BT_E_SIZE, "bt_trace_plan: negative niter");
     if (in.n == 0) return 0;
     const int need_w = (plan.flags & BT_PLAN_NO_REFRACT) ? 0 : 1;
-    const bool analytic = (plan.flags & BT_PLAN_ANALYTIC) != 0;
+    const int kinds = (plan.flags & BT_PLAN_ANALYTIC) ? kKindsAnalytic
+                      : (plan.flags & BT_PLAN_SUM_APB) ? kKindsSumAPB : kKindsAll;
     const int mode = (plan.flags & BT_PLAN_MLP_START) ? 2 : (plan.flags & BT_PLAN_FAST_START) ? 1 : 0;
     const dim3 grid(grid_for(in.n)), block(kBlock);
     hipStream_t st = as_stream(stream);
-    switch (mode * 2 + (analytic ? 1 : 0)) {
-    case 0: hipLaunchKernelGGL((k_plan<0, kKindsAll>), grid, block, 0, st, plan, in, out, niter, need_w); break;
-    case 1: hipLaunchKernelGGL((k_plan<0, kKindsAnalytic>), grid, block, 0, st, plan, in, out, niter, need_w); break;
-    case 2: hipLaunchKernelGGL((k_plan<1, kKindsAll>), grid, block, 0, st, plan, in, out, niter, need_w); break;
-    case 3: hipLaunchKernelGGL((k_plan<1, kKindsAnalytic>), grid, block, 0, st, plan, in, out, niter, need_w); break;
-    case 4: hipLaunchKernelGGL((k_plan<2, kKindsAll>), grid, block, 0, st, plan, in, out, niter, need_w); break;
-    default: hipLaunchKernelGGL((k_plan<2, kKindsAnalytic>), grid, block, 0, st, plan, in, out, niter, need_w); break;
+#define BT_PLAN_LAUNCH(M, K) \
+    hipLaunchKernelGGL((k_plan<M, K>), grid, block, 0, st, plan, in, out, niter, need_w)
+    switch (mode * 3 + kinds) {
+    case 0: BT_PLAN_LAUNCH(0, kKindsAll); break;
+    case 1: BT_PLAN_LAUNCH(0, kKindsAnalytic); break;
+    case 2: BT_PLAN_LAUNCH(0, kKindsSumAPB); break;
+    case 3: BT_PLAN_LAUNCH(1, kKindsAll); break;
+    case 4: BT_PLAN_LAUNCH(1, kKindsAnalytic); break;
+    case 5: BT_PLAN_LAUNCH(1, kKindsSumAPB); break;
+    case 6: BT_PLAN_LAUNCH(2, kKindsAll); break;
+    case 7: BT_PLAN_LAUNCH(2, kKindsAnalytic); break;
+    default: BT_PLAN_LAUNCH(2, kKindsSumAPB); break;
     }
+#undef BT_PLAN_LAUNCH
     return check_launch("bt_trace_plan");
 }
 

@@ -57,6 +57,28 @@ def start_key(start):
     return start
 
 
+_APB = (_lib.ASPHERE, _lib.POLYNOMIAL, _lib.BICUBIC)
+
+
+def _sums_are_apb(recs):
+    """True when every record is an analytic kind or belongs to a Sum of
+    exactly (Asphere, Polynomial, Bicubic) children, in that order: the
+    layout the trace's kKindsSumAPB kernel unrolls."""
+    i = 0
+    while i < len(recs):
+        r = recs[i]
+        if r.kind == _lib.SUM:
+            kids = tuple(c.kind for c in recs[i + 1:i + 1 + r.nchild])
+            if kids != _APB:
+                return False
+            i += 1 + r.nchild
+        elif r.kind in _lib.ANALYTIC_KINDS:
+            i += 1
+        else:
+            return False
+    return True
+
+
 class PlanSpec:
     """Host-side plan: ctypes records plus the numpy tables they point to."""
 
@@ -124,7 +146,9 @@ class PlanSpec:
         self.flags = ((0 if refracts else _lib.PLAN_NO_REFRACT)
                       | (_lib.PLAN_FAST_START if fast else 0)
                       | (_lib.PLAN_MLP_START if mlp else 0)
-                      | (_lib.PLAN_ANALYTIC if analytic else 0))
+                      | (_lib.PLAN_ANALYTIC if analytic else 0)
+                      | (_lib.PLAN_SUM_APB if not analytic and _sums_are_apb(self.surf_recs)
+                         else 0))
 
 
 class DevicePlan:

@@ -158,10 +158,17 @@ enum bt_action { BT_ACT_INTERSECT = 0, BT_ACT_REFLECT = 1, BT_ACT_REFRACT = 2 };
  *              bit-identical to the reference;
  *   CONIC      the analytic root of the base conic (Asphere's quadric, or a
  *              Sum's first child), then Newton on the full surface until the
- *              sag gap is <= 1e-14 m (at most 8 iterations);
+ *              sag gap is <= 1e-14 m (at most niter iterations);
  *   MLP        as CONIC, plus the learned correction of the step's network.
- * A ray that does not converge from a fast start is re-intersected on the
- * REFERENCE path, so failure semantics stay the reference's.  Starts other
+ * A ray that does not converge from a fast start within niter iterations is
+ * re-intersected on the REFERENCE path, which decides its flags.  Where the
+ * fast start is the reference's own start (CONIC on an Asphere: the same
+ * iterates), a ray the reference fails is therefore failed here too.  From
+ * a different start (a Sum's first-child root, an MLP prediction) a fast
+ * start can converge on a ray whose reference iteration would not have
+ * converged in niter steps: that ray is intersected where the reference
+ * marks it failed.  Positions agree with the reference to <= 1e-12 m where
+ * both intersect.  Starts other
  * than REFERENCE are honoured only by bt_trace_plan with BT_PLAN_FAST_START
  * set; MLP starts only with BT_PLAN_MLP_START also set, and an MLP step with
  * a null network runs as CONIC. */
@@ -189,6 +196,15 @@ typedef struct bt_step {
                                   the trace runs a kernel without the
                                   Bicubic, polynomial and Sum code (fewer
                                   registers); under this flag any other kind
+                                  evaluates to NaN and fails              */
+#define BT_PLAN_SUM_APB   16   /* (without BT_PLAN_ANALYTIC) every surface
+                                  record is one of the kinds above or a Sum
+                                  of exactly (Asphere, Polynomial, Bicubic)
+                                  children in that order -- a perturbed
+                                  conic: the trace runs a kernel with those
+                                  three children unrolled; under this flag
+                                  a Bicubic or Polynomial outside such a
+                                  Sum, or a Sum of another layout,
                                   evaluates to NaN and fails              */
 
 typedef struct bt_plan {

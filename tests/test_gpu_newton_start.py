@@ -227,3 +227,31 @@ def test_fast_start_ragged_sizes_and_partial_networks(trained):
         ok = ~a["failed"]
         for c in ("x", "y", "z", "t"):
             assert np.abs(a[c][ok] - b[c][ok]).max(initial=0) <= 1e-12
+
+
+def test_conic_start_keeps_the_reference_iteration_budget():
+    """Rays whose reference Newton (same start, same iterates) has not
+    converged after niter = 5 iterations: the conic start stops there too
+    and hands them to the reference path, so their flags are the
+    reference's (failed), not intersected by a longer iteration
+    (test_oracle_golden.slow_newton_case: 5 iterations fail thousands, 6
+    intersect all)."""
+    from test_oracle_golden import slow_newton_case
+    surf, rays = slow_newton_case()
+    optic = bt.CompoundOptic([bt.Interface(surf, name="S")], name="slow",
+                             inMedium=bt.vacuum)
+    base = bt.RayVector(**{k: rays[k] for k in ("x", "y", "z", "vx", "vy", "vz", "t",
+                                                 "wavelength", "flux")},
+                        vignetted=rays["vignetted"], failed=rays["failed"])
+    ref = base.copy()
+    optic.trace(ref)
+    fast = base.copy()
+    optic.trace(fast, start="conic")
+    torch.cuda.synchronize()
+    assert ref.failed.sum() > 500
+    np.testing.assert_array_equal(fast.failed, ref.failed)
+    np.testing.assert_array_equal(fast.vignetted, ref.vignetted)
+    ok = ~ref.failed
+    for k in ("x", "y", "z", "t"):
+        np.testing.assert_allclose(getattr(fast, k)[ok], getattr(ref, k)[ok], rtol=0,
+                                   atol=1e-12)

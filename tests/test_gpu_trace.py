@@ -11,6 +11,7 @@ import torch
 import batoid_amd as bt
 import oracle
 from oracle.host import HostRays, HostPlan, host_records, nthreads
+from batoid_amd import _lib
 from batoid_amd.plan import PlanSpec, collect_steps
 from helpers import (surfaces, media, obscurations, random_rays, transforms,
                      assert_cols_identical, FCOLS, BCOLS)
@@ -259,3 +260,51 @@ def test_perturbed_tma_bit_exact():
         got = _host(rv)
         assert_cols_identical(got, ref.cols, what=f"perturbed TMA field {k}")
         assert 0 < got["vignetted"].sum() < len(rv)
+
+
+def test_plan_kernels_agree_on_sum_layouts():
+    """The perturbed TMA's plan runs the kernel with its Sum(Asphere,
+    Polynomial, Bicubic) mirrors unrolled; the all-kinds kernel on the same
+    plan gives the same bits.  The freeform TMA (Sum(Polynomial, Bicubic))
+    runs the all-kinds kernel, bit-identical to the oracle."""
+    from batoid_amd import workloads
+    from batoid_amd import _device
+    optic = workloads.perturbed_tma()
+    thx, thy = workloads.tma_fields()
+    rv = workloads.tma_rays(optic, thx[3], thy[3], 770e-9, nrad=60, naz=600)
+    plan = optic._plan(rv.coordSys, False)
+    assert plan.struct.flags & _lib.PLAN_SUM_APB
+    a = rv.copy()
+    optic.trace(a)
+    b = rv.copy()
+    st = _lib.BtPlan.from_buffer_copy(plan.struct)
+    st.flags &= ~_lib.PLAN_SUM_APB
+    _lib.call("bt_trace_plan", st, b._rays, b._rays, NITER, _device.stream())
+    assert_cols_identical(_host(b), _host(a), what="all-kinds vs APB kernel")
+
+    free = workloads.freeform_tma()
+    rv = workloads.tma_rays(free, thx[3], thy[3], 770e-9, nrad=60, naz=600)
+    assert not free._plan(rv.coordSys, False).struct.flags & _lib.PLAN_SUM_APB
+    ref = HostRays.from_rv(rv)
+    HostPlan(PlanSpec(collect_steps(free), rv.coordSys)).trace(ref, niter=NITER)
+    free.trace(rv)
+    assert_cols_identical(_host(rv), ref.cols, what="freeform TMA")
+
+
+def test_sum_layout_flag_on_other_sums_fails_rays():
+    """A plan flagged BT_PLAN_SUM_APB whose Sum has another layout fails the
+    rays that reach it (the kernel checks the layout before reading the
+    children), rather than reading the wrong records."""
+    from batoid_amd import _device
+    optic = bt.CompoundOptic([
+        bt.Mirror(bt.Sum(bt.Plane(), bt.Paraboloid(12.0)), name="M"),
+        bt.Detector(bt.Plane(), name="D", coordSys=bt.CoordSys(origin=[0, 0, 3.0]))],
+        name="sum2", inMedium=bt.vacuum)
+    base = HostRays(**random_rays(4096, seed=23, z0=2.0, spread=0.02, prefail=0.0))
+    rv = _gpu_rays(base)
+    plan = optic._plan(rv.coordSys, False)
+    assert not plan.struct.flags & (_lib.PLAN_SUM_APB | _lib.PLAN_ANALYTIC)
+    st = _lib.BtPlan.from_buffer_copy(plan.struct)
+    st.flags |= _lib.PLAN_SUM_APB
+    _lib.call("bt_trace_plan", st, rv._rays, rv._rays, NITER, _device.stream())
+    assert rv.failed.all() and rv.vignetted.all()

@@ -139,7 +139,17 @@ def test_yaml_and_plan():
         _lib.PLAN_ANALYTIC | _lib.PLAN_FAST_START)
     # a Sum (or Bicubic / Zernike) surface needs the all-kinds kernel
     pert = workloads.perturbed_tma()
-    assert PlanSpec(collect_steps(pert), bt.CoordSys()).flags == 0
+    assert PlanSpec(collect_steps(pert), bt.CoordSys()).flags == _lib.PLAN_SUM_APB
+    # other Sum layouts (and Bicubic or polynomial leaves) need the all-kinds kernel
+    free = workloads.freeform_tma()
+    assert PlanSpec(collect_steps(free), bt.CoordSys()).flags == 0
+    from batoid_amd.plan import _sums_are_apb
+    a, p, b = pert['M1'].surface.surfaces
+    for surf, want in ((bt.Sum(a, p, b), True), (bt.Sum(a, b, p), False),
+                       (bt.Sum(p, a, b), False), (bt.Sum(a, p), False),
+                       (bt.Sum(a, p, b, bt.Plane()), False), (b, False), (p, False),
+                       (a, True), (bt.Sphere(3.0), True)):
+        assert _sums_are_apb(surf._records()[0]) == want, surf
     assert len(spec.med_recs) == 2                     # silica and vacuum, shared
     assert isinstance(optic['L2_exit'].surface, bt.Asphere)
     assert isinstance(optic['M1'].obscuration, bt.ObscNegation)

@@ -282,3 +282,30 @@ def test_huygens_oracle_obscured_airy():
                          2 * j1(u) / u - eps**2 * 2 * j1(eps * u) / (eps * u))
     model = model ** 2 / (1 - eps**2) ** 2
     np.testing.assert_allclose(I, model, rtol=0, atol=3e-3)
+
+
+def slow_newton_case():
+    """A strongly aspheric surface and oblique rays for which the reference's
+    tangent-plane Newton from the quadric root (asphere.cpp:75-85,
+    surface.cpp:22-49) needs more than niter = 5 iterations: a fast start
+    that iterated further than the reference would intersect rays the
+    reference marks failed."""
+    surf = bt.Asphere(3.0, -0.5, [0.05, 0.01])
+    rays = random_rays(20000, seed=1, z0=2.0, spread=0.8, radius=1.2, prefail=0.0,
+                       previg=0.0)
+    return surf, rays
+
+
+def test_slow_newton_case_needs_more_than_niter():
+    """The reference fails thousands of these rays at niter = 5 and
+    intersects every one of them at niter = 6 or 8."""
+    surf, rays = slow_newton_case()
+    tab = host_records(surf)
+    failed = {}
+    for niter in (5, 6, 8):
+        h = HostRays(**rays)
+        oracle.load().or_intersect(tab.ptr(), _dv(np.zeros(3)), _dv(np.eye(3)), h.struct,
+                                   niter, 1)
+        failed[niter] = h["failed"].astype(bool)
+    assert failed[5].sum() > 500
+    assert not failed[6].any() and not failed[8].any()
