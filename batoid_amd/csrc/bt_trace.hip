@@ -177,24 +177,9 @@ __global__ __launch_bounds__(kBlock) void k_obscure(
     if (obsc_contains(obsc, obsc_len, rv.x[i], rv.y[i])) rv.vignetted[i] = 1;
 }
 
-// Media indices evaluated once per ray (at most kPlanMedia per plan).
-constexpr int kPlanMedia = 4;
-
-__device__ __forceinline__ double pick_medium(const double (&nm)[kPlanMedia], int m)
-{
-    // m is wave-uniform: a scalar branch, no dynamic register indexing
-    switch (m) {
-    case 0: return nm[0];
-    case 1: return nm[1];
-    case 2: return nm[2];
-    default: return nm[3];
-    }
-}
-
 // Every step of a plan applied while the ray stays in registers.  A ray's
-// wavelength never changes, so the index of each of the plan's media is
-// evaluated once per ray up front (at most kPlanMedia media; larger plans
-// evaluate the medium at each refraction).
+// wavelength never changes, so the index of the plan's (first) dispersive
+// medium is evaluated once per ray up front.
 //
 // Mode 0: reference starts only.  Mode 1: CONIC starts honoured (an MLP step
 // runs as CONIC).  Mode 2: MLP starts too; MLP inference is a wave-wide MFMA
@@ -267,21 +252,16 @@ void k_plan(
     r.x = in.x[i]; r.y = in.y[i]; r.z = in.z[i];
     r.vx = in.vx[i]; r.vy = in.vy[i]; r.vz = in.vz[i];
     r.t = in.t[i];
-    r.w = need_w ? in.wavelength[i] : 0.0;
+    // Refractive indices: a constant medium's is its record's (a scalar
+    // load); the plan's first dispersive medium is evaluated once per ray and
+    // kept; any other dispersive medium is evaluated at its refraction, with
+    // the wavelength read again from memory (it is not held in registers).
     const int nmed = plan.nmedia;
-    const bool cached = nmed <= kPlanMedia;
-    double nm[kPlanMedia] = {0.0, 0.0, 0.0, 0.0};
-    if (need_w && cached && !r.fail) {
-        for (int m = 0; m < nmed; m++) {
-            const double v = medium_n(plan.media + m, r.w);
-            switch (m) {
-            case 0: nm[0] = v; break;
-            case 1: nm[1] = v; break;
-            case 2: nm[2] = v; break;
-            default: nm[3] = v; break;
-            }
-        }
-    }
+    int cm = -1;                               // the cached medium (wave-uniform)
+    for (int m = 0; m < nmed; m++)
+        if (plan.media[m].kind != BT_MED_CONST) { cm = m; break; }
+    double ncm = 0.0;
+    if (need_w && cm >= 0 && !r.fail) ncm = medium_n(plan.media + cm, in.wavelength[i]);
     const int nstep = plan.nstep;
     for (int k = 0; k < nstep; k++) {
         if (!Mlp && r.fail && r.vig) break;    // nothing below can change this ray
@@ -291,7 +271,9 @@ void k_plan(
         double n2 = 0.0;
         if (action == BT_ACT_REFRACT && !r.fail) {
             const int mi = st->medium;
-            n2 = cached ? pick_medium(nm, mi) : medium_n(plan.media + mi, r.w);
+            const bt_medium* med = plan.media + mi;
+            n2 = (med->kind == BT_MED_CONST) ? med->p[0]           // medium.cpp:30-32
+                 : (mi == cm) ? ncm : medium_n(med, in.wavelength[i]);
         }
         double correction = 0.0;
         int start = Fast ? st->start : BT_START_REFERENCE;

@@ -39,6 +39,10 @@ MEASURED_TRAFFIC_PER_RAY = 124.05
 # (ADD + MUL + 2 FMA) FP64 wave instructions (SQ_INSTS_VALU_*_F64), same file
 # (current build's figure).
 FP64_FLOP_PER_RAY = 6942.0
+# The same two figures for the C3 kernel (perturbed TMA, reference start),
+# from profiles/trace_plan_c3_counters.md (None until measured).
+C3_FP64_FLOP_PER_RAY = None
+C3_TRAFFIC_PER_RAY = None
 # Ray generation writes 9 FP64 columns and 2 flag bytes per ray.
 RAYGEN_BYTES_PER_RAY = 9 * 8 + 2
 
@@ -47,32 +51,67 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(optic, rv, target_s=10.0):
-    """Time the reference's CPU trace (its own C++ core, oracle/_ref, one call
-    per Interface exactly as batoid's Optic.trace issues them) -- or the C
-    restatement when _ref is absent -- on a strided sample of the batch."""
+def host_cpu():
+    """The host CPU the baselines run on: threads used (every CPU this
+    process may run on, what `nproc` reports), the model name, the logical
+    CPU count and the cgroup CPU quota where one is set."""
+    threads = len(os.sched_getaffinity(0))
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, period = f.read().split()
+            if q != "max":
+                quota = float(q) / float(period)
+    except (OSError, ValueError):
+        pass
+    return {"threads": threads, "model": model, "logical_cpus": os.cpu_count(),
+            "cgroup_cpu_quota": quota, "omp_num_threads_env": os.environ.get("OMP_NUM_THREADS")}
+
+
+def median_of_runs(run, runs=5):
+    """One warm-up call of run(), then `runs` timed calls; returns the median
+    and all times in seconds."""
+    run()
+    times = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        run()
+        times.append(time.perf_counter() - t0)
+    return float(np.median(times)), times
+
+
+def cpu_trace(optic, rv, what, run_s=2.0, runs=5):
+    """The reference's CPU trace (its own C++ core, oracle/_ref, one call per
+    Interface exactly as batoid's Optic.trace issues them; the oracle's C
+    restatement where _ref is absent) with every host CPU, on a strided
+    sample of `rv` sized so one run takes about run_s: the median of `runs`
+    runs after a warm-up, in ray-surface intersections/s."""
     import oracle
-    from oracle.host import HostRays, host_records, nthreads
+    from oracle.host import HostRays, host_records, _HostTable
     from batoid_amd.plan import PlanSpec, collect_steps
-    threads = nthreads()
+    cpu = host_cpu()
+    threads = cpu["threads"]
     use_ref = oracle.ref_available()
     lib = oracle.load_ref() if use_ref else oracle.load()
     spec = PlanSpec(collect_steps(optic), rv.coordSys)
     tabs = [host_records(i.surface) for i in spec.interfaces]
     obscs = [host_records(i.obscuration) if i.obscuration is not None else None
              for i in spec.interfaces]
-    meds = {}
-    for st in spec.steps:
-        if st.medium >= 0:
-            from oracle.host import _HostTable
-            meds[st.medium] = _HostTable([spec.med_recs[st.medium]], [spec.med_arrs[st.medium]])
+    meds = {st.medium: _HostTable([spec.med_recs[st.medium]], [spec.med_arrs[st.medium]])
+            for st in spec.steps if st.medium >= 0}
     full = HostRays.from_rv(rv)
     pre = "ref_" if use_ref else "or_"
 
-    def run(n):
-        idx = np.linspace(0, len(full) - 1, n).astype(np.int64)
-        h = HostRays(**{k: v[idx] for k, v in full.cols.items()})
-        t0 = time.perf_counter()
+    def trace(h):
         for st, tab, ob in zip(spec.steps, tabs, obscs):
             dr = (ctypes.c_double * 3)(*st.dr)
             drot = (ctypes.c_double * 9)(*st.drot)
@@ -84,21 +123,92 @@ def cpu_baseline(optic, rv, target_s=10.0):
                 getattr(lib, pre + name)(tab.ptr(), dr, drot, h.struct, 5, threads)
             if ob is not None:
                 getattr(lib, pre + "obscure")(ob.ptr(), ob.count, h.struct, threads)
-        return time.perf_counter() - t0
 
-    t_small = run(20000)
-    n = int(min(len(full), max(20000, 20000 * target_s / max(t_small, 1e-6))))
-    t = run(n)
-    value = n * len(spec.steps) / t
+    def make(n, reps=1):
+        """A run: `reps` traces of a strided n-ray sample, each from fresh
+        input columns (restored outside the timed calls); returns the time
+        spent in the trace calls."""
+        idx = np.linspace(0, len(full) - 1, n).astype(np.int64)
+        base = {k: v[idx] for k, v in full.cols.items()}
+        h = HostRays(**base)
+
+        def run():
+            total = 0.0
+            for _ in range(reps):
+                for k, v in base.items():
+                    np.copyto(h.cols[k], v)
+                t0 = time.perf_counter()
+                trace(h)
+                total += time.perf_counter() - t0
+            return total
+        return run
+
+    def median(run, count):
+        run()                                     # warm-up
+        times = [run() for _ in range(count)]
+        return float(np.median(times)), times
+
+    # size one run to about run_s: a strided sample of the batch, or the
+    # whole batch traced `reps` times over when it takes less than that
+    n = min(len(full), 20000)
+    t_small, _ = median(make(n), 1)
+    n = int(min(len(full), max(n, n * run_s / max(t_small, 1e-6))))
+    reps = 1
+    if n == len(full):
+        t_full, _ = median(make(n), 1)
+        reps = max(1, int(round(run_s / max(t_full, 1e-6))))
+    t, times = median(make(n, reps), runs)
+    value = reps * n * len(spec.steps) / t
+    log(f"cpu {what}: {value:.3e} ray-surfaces/s ({n} rays x {reps}, median of {runs}: "
+        f"{t:.2f} s, {threads} threads)")
     return {"value": value, "unit": "ray-surface intersections/s", "cores": threads,
             "kind": "reference" if use_ref else "port",
-            "sample": f"{n} of {len(full)} rays (strided) of the first batch, "
-                      f"{len(spec.steps)} interfaces, {t:.2f} s, "
-                      f"{'reference C++ core (oracle/_ref)' if use_ref else 'C restatement (oracle)'} "
-                      f"with {threads} OpenMP threads"}
+            "sample": f"{what}: {n} of {len(full)} rays (strided)"
+                      + (f", traced {reps} times per run" if reps > 1 else "")
+                      + f", {len(spec.steps)} interfaces, median of {runs} runs after a warm-up "
+                      f"({', '.join(f'{x:.2f}' for x in times)} s), "
+                      f"{'reference C++ core (oracle/_ref)' if use_ref else 'C restatement (oracle)'}"
+                      f" with {threads} OpenMP threads",
+           "host": cpu}
 
 
-def bench_huygens(optic, reps=10, nx=512):
+def cpu_huygens(rays, point, prim, npix, time_, sub=16, runs=5):
+    """The reference's huygensPSF image loop (analysis.py:302-303: one
+    sumAmplitude per pixel, rayVector.cpp:169-222, OpenMP over rays) through
+    its own C++ core, on a sub x sub block of the npix x npix lattice around
+    the centre; the time scales with pixels x rays, so s per PSF =
+    block time x (npix / sub)^2."""
+    import oracle
+    from oracle.host import HostRays
+    cpu = host_cpu()
+    threads = cpu["threads"]
+    use_ref = oracle.ref_available()
+    lib = oracle.load_ref() if use_ref else oracle.load()
+    fn = lib.ref_sum_amplitude if use_ref else lib.or_sum_amplitude
+    h = HostRays.from_rv(rays)
+    out = np.empty(2)
+    pts = [np.asarray(point[:2], float) + (i - sub // 2) * prim[0] + (j - sub // 2) * prim[1]
+           for j in range(sub) for i in range(sub)]
+
+    def run():
+        for p in pts:
+            fn(h.struct, float(p[0]), float(p[1]), 0.0, time_, 1, out.ctypes.data, threads)
+
+    t, times = median_of_runs(run, runs=runs)
+    per_psf = t * (npix / sub) ** 2
+    nsum = int((~rays.vignetted & ~rays.failed).sum())
+    log(f"cpu huygens: {per_psf:.1f} s per {npix}^2 PSF ({sub}^2 block, {threads} threads)")
+    return {"value": per_psf, "unit": "s per PSF", "cores": threads,
+            "kind": "reference" if use_ref else "port",
+            "tflops": 8.0 * npix * npix * nsum / per_psf / 1e12,
+            "sample": f"{sub}x{sub} of the {npix}x{npix} pixels (one sumAmplitude per pixel "
+                      f"over {nsum} summed rays, as the reference's huygensPSF loop), median "
+                      f"of {runs} runs after a warm-up ({', '.join(f'{x:.2f}' for x in times)}"
+                      f" s), scaled by ({npix}/{sub})^2",
+            "host": cpu}
+
+
+def bench_huygens(optic, reps=10, nx=512, cpu=False):
     """C4: the TMA pupil on an nx x nx grid (one field, 620 nm), traced, then
     a 256 x 256 Huygens PSF at 0.5 um pixels by the MFMA GEMM (FP32 and
     split-BF16) and by the FP64 direct sum.  TFLOP/s counts 8 flops per
@@ -159,6 +269,143 @@ def bench_huygens(optic, reps=10, nx=512):
     res["bf16x3"]["frac_of_bf16_mfma_peak"] = 3 * res["bf16x3"]["tflops"] / BF16_MFMA_PEAK_TFLOPS
     res["speedup_fp32_vs_fp64_direct"] = res["direct"]["ms_per_psf"] / res["fp32"]["ms_per_psf"]
     res["speedup_bf16x3_vs_fp64_direct"] = res["direct"]["ms_per_psf"] / res["bf16x3"]["ms_per_psf"]
+    if cpu:
+        res["cpu_baseline"] = cpu_huygens(rays, point, prim, npix, time_)
+        res["speedup_fp32_vs_cpu_reference"] = (res["cpu_baseline"]["value"] * 1e3
+                                                / res["fp32"]["ms_per_psf"])
+    return res
+
+
+def _out_structs(batches, dev):
+    """One output buffer for a list of input batches of equal size; each
+    step reads its batch and writes the buffer (wavelength and flux are
+    shared with the input).  Returns the columns and (in, out) bt_rays."""
+    import torch
+    from batoid_amd import _lib
+    n = len(batches[0])
+    cols = {k: torch.empty(n, dtype=torch.float64, device=dev)
+            for k in ("x", "y", "z", "vx", "vy", "vz", "t")}
+    cols.update({k: torch.empty(n, dtype=torch.uint8, device=dev)
+                 for k in ("vignetted", "failed")})
+    structs = []
+    for rv in batches:
+        o = _lib.BtRays()
+        for k, v in cols.items():
+            setattr(o, k, v.data_ptr())
+        o.wavelength = rv.torch("wavelength").data_ptr()
+        o.flux = rv.torch("flux").data_ptr()
+        o.n = n
+        structs.append((rv._rays, o))
+    return cols, structs
+
+
+def _time_plan(plan, structs, steps, warmup):
+    """Wall time of `steps` bt_trace_plan launches (cycling through the
+    batches) and the mean launch time by HIP events on the launch stream."""
+    import torch
+    from batoid_amd import _lib, _device
+    from batoid_amd.global_vars import _batoid_niter
+    lib = _lib.load()
+    stream = _device.stream()
+
+    def step(i):
+        rin, rout = structs[i % len(structs)]
+        rc = lib.bt_trace_plan(plan.struct, rin, rout, _batoid_niter, stream)
+        if rc:
+            _lib.check(rc, "bt_trace_plan")
+
+    for i in range(warmup):
+        step(i)
+    torch.cuda.synchronize()
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    t0 = time.perf_counter()
+    for i in range(steps):
+        starts[i].record()
+        step(i)
+        ends[i].record()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    kernel_ms = float(np.mean([a.elapsed_time(b) for a, b in zip(starts, ends)]))
+    return elapsed, kernel_ms
+
+
+def _plan_kernel_name(plan):
+    from batoid_amd import _lib
+    f = plan.struct.flags
+    mode = 2 if f & _lib.PLAN_MLP_START else (1 if f & _lib.PLAN_FAST_START else 0)
+    kinds = ("kKindsAnalytic" if f & _lib.PLAN_ANALYTIC
+             else "kKindsSumAPB" if f & _lib.PLAN_SUM_APB else "kKindsAll")
+    return f"k_plan<{mode}, {kinds}> (bt_trace_plan)"
+
+
+def bench_c1(steps=50, warmup=3, cpu=True):
+    """C1: the Newtonian telescope (Paraboloid mirror + flat detector at its
+    focus), 1e6 seeded rays from a uniform pupil at 0.1 deg, 500 nm: the
+    fused trace (reflect + obscure, intersect) from an input batch into an
+    output buffer.  HBM: reads x..t + 2 flags (58 B), writes the same
+    (58 B): 116 B per ray (no refraction, so no wavelength)."""
+    from batoid_amd import workloads, _device
+    optic = workloads.newtonian()
+    rv = workloads.newtonian_rays(optic)
+    n = len(rv)
+    plan = optic._plan(rv.coordSys, False)
+    _, structs = _out_structs([rv], _device.device())
+    elapsed, kernel_ms = _time_plan(plan, structs, steps, warmup)
+    nstep = plan.struct.nstep
+    gbs = 116.0 * n / (kernel_ms * 1e-3) / 1e9
+    res = {"workload": "Newtonian: Paraboloid(R=10 m) mirror, 0.2-1.0 m clear annulus, flat "
+                       "detector at z = 5 m; asPolar nrandom=1e6, seed 57721, 0.1 deg, 500 nm",
+           "rays": n, "interfaces": nstep, "kernel": _plan_kernel_name(plan),
+           "ms_per_trace": elapsed * 1e3 / steps, "kernel_ms": kernel_ms,
+           "value": n * nstep * steps / elapsed, "unit": "ray-surface intersections/s",
+           "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": gbs / HBM_PEAK_GBS, "bytes_per_ray": 116}}
+    log(f"C1: {elapsed * 1e3 / steps:.3f} ms per 1e6-ray trace (kernel {kernel_ms:.3f} ms), "
+        f"{res['value']:.3e} ray-surfaces/s")
+    if cpu:
+        res["cpu_baseline"] = cpu_trace(optic, rv, "C1 Newtonian")
+        res["speedup_vs_cpu_reference"] = res["value"] / res["cpu_baseline"]["value"]
+    return res
+
+
+def bench_c3(batches, steps=20, warmup=2, cpu=True):
+    """C3: the perturbed TMA (every mirror Sum(Asphere, Zernike j<=28,
+    Bicubic 204^2)) over the same resident pupils as C2, reference Newton
+    start (bit-identical to the reference), one (field, wavelength) batch
+    per step into an output buffer.  HBM bytes as C2 (124 B per ray); the
+    kernel is FP64 bound."""
+    from batoid_amd import workloads, _device
+    optic = workloads.perturbed_tma()
+    n = len(batches[0])
+    plan = optic._plan(batches[0].coordSys, False)
+    _, structs = _out_structs(batches, _device.device())
+    elapsed, kernel_ms = _time_plan(plan, structs, steps, warmup)
+    nstep = plan.struct.nstep
+    value = n * nstep * steps / elapsed
+    gbs = BYTES_PER_RAY * n / (kernel_ms * 1e-3) / 1e9
+    res = {"workload": "perturbed TMA: C2 with each mirror Sum(Asphere, Zernike j<=28 "
+                       "N(0, 50 nm), Bicubic 204^2 10 nm rms); the C2 pupils and fields",
+           "rays": n, "interfaces": nstep, "batches": len(batches),
+           "ms_per_step": elapsed * 1e3 / steps, "kernel_ms": kernel_ms, "value": value,
+           "unit": "ray-surface intersections/s", "start": "reference (bit-identical)"}
+    roof = {"kernel": _plan_kernel_name(plan), "bound": "fp64_valu"}
+    hbm = {"achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+           "bytes_per_ray": BYTES_PER_RAY}
+    if C3_FP64_FLOP_PER_RAY:
+        tf = C3_FP64_FLOP_PER_RAY * n / (kernel_ms * 1e-3) / 1e12
+        roof.update({"achieved": tf, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": tf / FP64_PEAK_TFLOPS, "flop_per_ray": C3_FP64_FLOP_PER_RAY})
+    if C3_TRAFFIC_PER_RAY:
+        roof["traffic"] = C3_TRAFFIC_PER_RAY * n
+        hbm["measured_bytes_per_ray"] = C3_TRAFFIC_PER_RAY
+    roof["hbm"] = hbm
+    res["roofline"] = roof
+    log(f"C3: {elapsed * 1e3 / steps:.3f} ms per step (kernel {kernel_ms:.3f} ms), "
+        f"{value:.3e} ray-surfaces/s")
+    if cpu:
+        res["cpu_baseline"] = cpu_trace(optic, batches[0], "C3 perturbed TMA, plain start")
+        res["speedup_vs_cpu_reference"] = value / res["cpu_baseline"]["value"]
     return res
 
 
@@ -586,6 +833,8 @@ def main():
     ap.add_argument("--no-raygen", action="store_true")
     ap.add_argument("--no-zernike", action="store_true")
     ap.add_argument("--no-split", action="store_true")
+    ap.add_argument("--no-c1", action="store_true")
+    ap.add_argument("--no-c3", action="store_true")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="also run a plain device copy of this many MiB (calibrates "
                          "FETCH_SIZE / WRITE_SIZE counters against known bytes)")
@@ -704,6 +953,7 @@ def main():
 
     bytes_per_launch = BYTES_PER_RAY * n
     achieved = bytes_per_launch / (kernel_ms * 1e-3) / 1e9
+    fp64_tflops = FP64_FLOP_PER_RAY * n / (kernel_ms * 1e-3) / 1e12
     result = {
         "metric": METRIC,
         "value": value,
@@ -725,24 +975,26 @@ def main():
             "fields": "first 6 of 189 uniform in a 1.75 deg disk",
             "niter": _batoid_niter, "unvignetted_fraction": round(frac_unvig, 4),
         },
+        # the kernel keeps each ray in registers across all 12 interfaces:
+        # 124 B of HBM per ray against ~7k FP64 flops, so FP64 VALU issue
+        # binds it, not HBM (the HBM figures are beside it)
         "roofline": {
-            "kernel": "k_plan<0, kKindsAnalytic> (bt_trace_plan)",
-            "bound": "hbm",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
+            "kernel": _plan_kernel_name(plan),
+            "bound": "fp64_valu",
+            "achieved": fp64_tflops,
+            "peak": FP64_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": fp64_tflops / FP64_PEAK_TFLOPS,
+            "flop_per_ray": FP64_FLOP_PER_RAY,
             "traffic": MEASURED_TRAFFIC_PER_RAY * n,
-            "bytes_per_launch": bytes_per_launch,
             "kernel_ms": kernel_ms,
-            # the kernel keeps rays in registers across all 12 interfaces and
-            # is FP64-VALU bound; its compute roofline:
-            "fp64": {
-                "achieved": FP64_FLOP_PER_RAY * n / (kernel_ms * 1e-3) / 1e12,
-                "peak": FP64_PEAK_TFLOPS,
-                "unit": "TFLOP/s",
-                "frac": FP64_FLOP_PER_RAY * n / (kernel_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
-                "flop_per_ray": FP64_FLOP_PER_RAY,
+            "hbm": {
+                "achieved": achieved,
+                "peak": HBM_PEAK_GBS,
+                "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS,
+                "bytes_per_launch": bytes_per_launch,
+                "measured_bytes_per_ray": MEASURED_TRAFFIC_PER_RAY,
             },
         },
     }
@@ -751,8 +1003,12 @@ def main():
         "max_abs_dpos_m": fast_dpos, "flag_mismatches": fast_flags,
         "note": "same workload, Newton surfaces seeded by their conic root and "
                 "converged to a 1e-14 m sag gap (not bit-identical to the reference)"}
+    if not args.no_c3:
+        result["c3"] = bench_c3(batches, cpu=not args.no_cpu)
+    if not args.no_c1:
+        result["c1"] = bench_c1(cpu=not args.no_cpu)
     if not args.no_huygens:
-        result["huygens"] = bench_huygens(optic)
+        result["huygens"] = bench_huygens(optic, cpu=not args.no_cpu)
         # the same PSF from a ~2e5-ray pupil (asGrid nx=640 over the annulus)
         result["huygens_nx640"] = bench_huygens(optic, nx=640)
     if not args.no_raygen:
@@ -769,8 +1025,8 @@ def main():
         result["batch"] = bench_batch()
     if not args.no_cpu:
         log("timing the CPU baseline ...")
-        result["cpu_baseline"] = cpu_baseline(optic, batches[0])
-        log(f"cpu baseline {result['cpu_baseline']['value']:.3e} ray-surfaces/s")
+        result["cpu_baseline"] = cpu_trace(optic, batches[0], "C2 TMA, first batch")
+        result["speedup_vs_cpu_reference"] = value / result["cpu_baseline"]["value"]
     print(json.dumps(result), flush=True)
 
 
