@@ -531,6 +531,88 @@ BT_DEV void surf_sag_normal(const bt_surface* s, double x, double y, double& sag
     sum_normal_from_slopes(sx, sy, nx, ny, nz);
 }
 
+// Surface evaluators for the Newton loops.  SurfEval<K> dispatches on the
+// record's kind at every evaluation; where the caller already knows the kind
+// (an Asphere leaf, a Sum laid out as kKindsSumAPB requires) the loop is
+// instantiated with that kind's functions directly, so it carries no switch
+// over the other kinds.  Every evaluator computes exactly what surf_sag /
+// surf_normal / surf_sag_normal compute for that record.
+template <int K>
+struct SurfEval {
+    static BT_DEV double sag(const bt_surface* s, double x, double y)
+    {
+        return surf_sag<K>(s, x, y);
+    }
+    static BT_DEV void normal(const bt_surface* s, double x, double y, double& nx, double& ny,
+                              double& nz)
+    {
+        surf_normal<K>(s, x, y, nx, ny, nz);
+    }
+    static BT_DEV void sag_normal(const bt_surface* s, double x, double y, double& sag,
+                                  double& nx, double& ny, double& nz)
+    {
+        surf_sag_normal<K>(s, x, y, sag, nx, ny, nz);
+    }
+};
+
+struct AsphereEval {                                 // asphere.cpp:37-62
+    static BT_DEV double sag(const bt_surface* s, double x, double y)
+    {
+        return asphere_sag(s, x, y);
+    }
+    static BT_DEV void normal(const bt_surface* s, double x, double y, double& nx, double& ny,
+                              double& nz)
+    {
+        conic_normal<true>(s, x, y, nx, ny, nz);
+    }
+    static BT_DEV void sag_normal(const bt_surface* s, double x, double y, double& sag,
+                                  double& nx, double& ny, double& nz)
+    {
+        sag = asphere_sag(s, x, y);
+        conic_normal<true>(s, x, y, nx, ny, nz);
+    }
+};
+
+// A Sum whose layout sum_is_apb has accepted (sum.cpp:23-45).
+struct SumAPBEval {
+    static BT_DEV double sag(const bt_surface* s, double x, double y)
+    {
+        double result = 0.0;
+        result += asphere_sag(s + 1, x, y);
+        result += poly_sag(s + 2, x, y);
+        result += table_eval(s + 3, x, y);
+        return result;
+    }
+    static BT_DEV void normal(const bt_surface* s, double x, double y, double& nx, double& ny,
+                              double& nz)
+    {
+        double sx = 0.0, sy = 0.0, tnx, tny, tnz;
+        conic_normal<true>(s + 1, x, y, tnx, tny, tnz);
+        add_slope(sx, sy, tnx, tny, tnz);
+        poly_normal(s + 2, x, y, tnx, tny, tnz);
+        add_slope(sx, sy, tnx, tny, tnz);
+        table_normal(s + 3, x, y, tnx, tny, tnz);
+        add_slope(sx, sy, tnx, tny, tnz);
+        sum_normal_from_slopes(sx, sy, nx, ny, nz);
+    }
+    static BT_DEV void sag_normal(const bt_surface* s, double x, double y, double& sag,
+                                  double& nx, double& ny, double& nz)
+    {
+        double result = 0.0, sx = 0.0, sy = 0.0, ts, tnx, tny, tnz;
+        result += asphere_sag(s + 1, x, y);
+        conic_normal<true>(s + 1, x, y, tnx, tny, tnz);
+        add_slope(sx, sy, tnx, tny, tnz);
+        result += poly_sag(s + 2, x, y);
+        poly_normal(s + 2, x, y, tnx, tny, tnz);
+        add_slope(sx, sy, tnx, tny, tnz);
+        table_sag_normal(s + 3, x, y, ts, tnx, tny, tnz);
+        result += ts;
+        add_slope(sx, sy, tnx, tny, tnz);
+        sag = result;
+        sum_normal_from_slopes(sx, sy, nx, ny, nz);
+    }
+};
+
 BT_DEV bool same_bits(double a, double b)
 {
     return __double_as_longlong(a) == __double_as_longlong(b);
@@ -552,7 +634,7 @@ struct NormalMemo {
     bool ok;
 };
 
-template <int K = kKindsAll>
+template <class Ev>
 BT_DEV bool newton(const bt_surface* s, double x, double y, double z,
                    double vx, double vy, double vz, double& dt, int niter,
                    NormalMemo* memo = nullptr)
@@ -560,11 +642,11 @@ BT_DEV bool newton(const bt_surface* s, double x, double y, double z,
     double rPx = x+vx*dt;
     double rPy = y+vy*dt;
     double rPz = z+vz*dt;
-    double sz = surf_sag<K>(s, rPx, rPy);
+    double sz = Ev::sag(s, rPx, rPy);
     double pdt = 0.0;            // the iterate before the current one (iter >= 1)
     for (int iter = 0; iter < niter; iter++) {
         double nx, ny, nz;
-        surf_normal<K>(s, rPx, rPy, nx, ny, nz);
+        Ev::normal(s, rPx, rPy, nx, ny, nz);
         if (memo) *memo = NormalMemo{dt, nx, ny, nz, true};
         double dtn = (rPx-x)*nx + (rPy-y)*ny + (sz-z)*nz;
         dtn /= (nx*vx + ny*vy + nz*vz);
@@ -574,7 +656,7 @@ BT_DEV bool newton(const bt_surface* s, double x, double y, double z,
         rPx = x+vx*dt;
         rPy = y+vy*dt;
         rPz = z+vz*dt;
-        sz = surf_sag<K>(s, rPx, rPy);
+        sz = Ev::sag(s, rPx, rPy);
         if (iter >= 1 && same_bits(dt, pdt)) {
             // two-cycle: iterates alternate between this state and the
             // previous one; after the remaining (niter-1-iter) iterations the
@@ -671,7 +753,7 @@ BT_DEV bool leaf_tti(const bt_surface* s, double x, double y, double z,
         return quadric_tti(s, x, y, z, vx, vy, vz, dt);
     case BT_ASPHERE:                                 // asphere.cpp:75-85
         if (!quadric_tti(s, x, y, z, vx, vy, vz, dt)) return false;
-        return newton<K>(s, x, y, z, vx, vy, vz, dt, niter, memo);
+        return newton<AsphereEval>(s, x, y, z, vx, vy, vz, dt, niter, memo);
     case BT_TILTED: {                                // tilted.cpp:29-37
         double num = z - x*s->p[0] - y*s->p[1];
         double den = vx*s->p[0] + vy*s->p[1] - vz;
@@ -681,7 +763,8 @@ BT_DEV bool leaf_tti(const bt_surface* s, double x, double y, double z,
     }
     case BT_BICUBIC:
     case BT_POLYNOMIAL:                              // surface.cpp:22-49
-        if constexpr (K == kKindsAll) return newton<K>(s, x, y, z, vx, vy, vz, dt, niter, memo);
+        if constexpr (K == kKindsAll)
+            return newton<SurfEval<K>>(s, x, y, z, vx, vy, vz, dt, niter, memo);
         break;
     }
     return false;
@@ -698,7 +781,10 @@ BT_DEV bool surf_tti(const bt_surface* s, double x, double y, double z,
     if (K == kKindsSumAPB && !sum_is_apb(s)) return false;
     // sum.cpp:47-56: first child's intersection seeds Newton on the Sum
     if (!leaf_tti<L>(s + 1, x, y, z, vx, vy, vz, dt, niter)) return false;
-    return newton<K>(s, x, y, z, vx, vy, vz, dt, niter, memo);
+    if constexpr (K == kKindsSumAPB)
+        return newton<SumAPBEval>(s, x, y, z, vx, vy, vz, dt, niter, memo);
+    else
+        return newton<SurfEval<K>>(s, x, y, z, vx, vy, vz, dt, niter, memo);
 }
 
 // The surface's analytic base start: the root of a Sum's first child or of an
@@ -740,14 +826,14 @@ constexpr double kFastTol = 1e-14;   // sag gap accepted by the fast starts [m]
 // the reference path, which decides it).  At 1e-13 m the residual position
 // errors, carried through a three-mirror optic, reach ~1e-14 in the final
 // direction cosines; 1e-14 m keeps them an order of magnitude below that.
-template <int K = kKindsAll>
+template <class Ev>
 BT_DEV bool newton_fast(const bt_surface* s, double x, double y, double z,
                         double vx, double vy, double vz, double& dt, NormalMemo* memo,
                         int max_iter)
 {
     double rPx = x+vx*dt, rPy = y+vy*dt, rPz = z+vz*dt;
     double sz, nx, ny, nz;
-    surf_sag_normal<K>(s, rPx, rPy, sz, nx, ny, nz);
+    Ev::sag_normal(s, rPx, rPy, sz, nx, ny, nz);
     for (int it = 0; it < max_iter; it++) {
         if (fabs(sz-rPz) <= kFastTol) break;
         double d = (rPx-x)*nx + (rPy-y)*ny + (sz-z)*nz;
@@ -755,7 +841,7 @@ BT_DEV bool newton_fast(const bt_surface* s, double x, double y, double z,
         rPx = x+vx*dt;
         rPy = y+vy*dt;
         rPz = z+vz*dt;
-        surf_sag_normal<K>(s, rPx, rPy, sz, nx, ny, nz);
+        Ev::sag_normal(s, rPx, rPy, sz, nx, ny, nz);
     }
     if (!(fabs(sz-rPz) <= kFastTol)) return false;
     *memo = NormalMemo{dt, nx, ny, nz, true};
@@ -775,7 +861,17 @@ BT_DEV bool fast_tti(const bt_surface* s, double x, double y, double z, double v
         double d0;
         const bool ok = conic_start<K>(s, x, y, z, vx, vy, vz, d0);
         d0 += correction;
-        if (ok && d0 == d0 && newton_fast<K>(s, x, y, z, vx, vy, vz, d0, memo, niter)) {
+        bool conv = false;
+        if (ok && d0 == d0) {
+            if (s->kind == BT_ASPHERE)
+                conv = newton_fast<AsphereEval>(s, x, y, z, vx, vy, vz, d0, memo, niter);
+            else if constexpr (K == kKindsSumAPB)
+                conv = sum_is_apb(s) &&
+                       newton_fast<SumAPBEval>(s, x, y, z, vx, vy, vz, d0, memo, niter);
+            else
+                conv = newton_fast<SurfEval<K>>(s, x, y, z, vx, vy, vz, d0, memo, niter);
+        }
+        if (conv) {
             dt = d0;
             return true;
         }

@@ -37,12 +37,12 @@ BYTES_PER_RAY = 8 * 8 + 2 + 7 * 8 + 2
 MEASURED_TRAFFIC_PER_RAY = 124.05
 # FP64 flops per ray of the same kernel on this workload: 64 lanes x
 # (ADD + MUL + 2 FMA) FP64 wave instructions (SQ_INSTS_VALU_*_F64), same file
-# (current build's figure).
-FP64_FLOP_PER_RAY = 6942.0
+# (current build's figure; issued lanes, an upper bound on useful flops).
+FP64_FLOP_PER_RAY = 6457.6
 # The same two figures for the C3 kernel (perturbed TMA, reference start),
-# from profiles/trace_plan_c3_counters.md (None until measured).
-C3_FP64_FLOP_PER_RAY = None
-C3_TRAFFIC_PER_RAY = None
+# from profiles/trace_plan_c3_perturbed_tma_1e7.md.
+C3_FP64_FLOP_PER_RAY = 19839.0
+C3_TRAFFIC_PER_RAY = 124.87
 # Ray generation writes 9 FP64 columns and 2 flag bytes per ray.
 RAYGEN_BYTES_PER_RAY = 9 * 8 + 2
 
