@@ -35,14 +35,16 @@ typedef float f16v __attribute__((ext_vector_type(16)));
 //   w1:   [2 rb][64 lanes][8] half  = [W1[n][0..5], b1[n], 0], n = 32 rb + lane%32
 //   w2:   [2 rb2][4 s][64 lanes][8] half = W2[o][sigma(s, lane/32, e)],
 //         o = 32 rb2 + lane%32, sigma(2 rb1 + t, h, e) = 32 rb1 + 16 t + 8 (e/4) + 4 h + e%4
-//   b2:   [2 rb2][64 lanes][16] float = b2[32 rb2 + 8 (e/4) + 4 (lane/32) + e%4]
-//   w3:   [2 rb2][64 lanes][16] float, same row map as b2
+//   b2:   [64] float; lane element e of row block rb2 reads
+//         b2[32 rb2 + 8 (e/4) + 4 (lane/32) + e%4]
+//   w3:   [64] float, read like b2
+// 11,328 bytes: small enough that the trace keeps a plan's networks in LDS.
 struct MlpPacked {
     float norm[16];
     _Float16 w1[2 * 64 * 8];
     _Float16 w2[2 * 4 * 64 * 8];
-    float b2[2 * 64 * 16];
-    float w3[2 * 64 * 16];
+    float b2[kMlpHidden];
+    float w3[kMlpHidden];
 };
 
 // Row of element e of a 32x32 MFMA accumulator held by lane half h.
@@ -87,16 +89,20 @@ __device__ __forceinline__ float mlp_predict_wave(const MlpPacked* __restrict__ 
     // layers 2 and 3, one output row block at a time
     for (int rb2 = 0; rb2 < 2; rb2++) {
         f16v acc0, acc1;
-        const float* b2 = P->b2 + (rb2 * 64 + lane) * 16;
-        for (int e = 0; e < 16; e++) { acc0[e] = b2[e]; acc1[e] = b2[e]; }
+        const float* b2 = P->b2 + 32 * rb2;
+        for (int e = 0; e < 16; e++) {
+            const float b = b2[acc_row(e, h)];
+            acc0[e] = b;
+            acc1[e] = b;
+        }
         for (int s = 0; s < 4; s++) {
             const h8 a = *reinterpret_cast<const h8*>(P->w2 + ((rb2 * 4 + s) * 64 + lane) * 8);
             acc0 = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, hb[0][s >> 1][s & 1], acc0, 0, 0, 0);
             acc1 = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, hb[1][s >> 1][s & 1], acc1, 0, 0, 0);
         }
-        const float* w3 = P->w3 + (rb2 * 64 + lane) * 16;
+        const float* w3 = P->w3 + 32 * rb2;
         for (int e = 0; e < 16; e++) {
-            const float w = w3[e];
+            const float w = w3[acc_row(e, h)];
             const float v0 = acc0[e], v1 = acc1[e];
             part[0] = __builtin_fmaf(w, v0 > 0.f ? v0 : 0.f, part[0]);
             part[1] = __builtin_fmaf(w, v1 > 0.f ? v1 : 0.f, part[1]);

@@ -187,12 +187,9 @@ __global__ __launch_bounds__(64) void k_mlp_pack(const float* __restrict__ P,
                 w2[e] = (_Float16)P[kMlpOffW2 + o * kMlpHidden + in];
             }
         }
-        for (int e = 0; e < 16; e++) {
-            const int row = 32 * rb2 + acc_row(e, h);
-            Q->b2[(rb2 * 64 + lane) * 16 + e] = P[kMlpOffB2 + row];
-            Q->w3[(rb2 * 64 + lane) * 16 + e] = P[kMlpOffW3 + row];
-        }
     }
+    Q->b2[lane] = P[kMlpOffB2 + lane];
+    Q->w3[lane] = P[kMlpOffW3 + lane];
 }
 
 // Packed (MFMA) inference, de-normalised; whole waves, padded lanes masked.

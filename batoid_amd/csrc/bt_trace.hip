@@ -192,7 +192,7 @@ __global__ __launch_bounds__(kBlock) void k_obscure(
 template <int K>
 __device__ __forceinline__ void mlp_features(const bt_step* __restrict__ st,
                                              const bt_surface* surf, const Ray& r,
-                                             float (&f)[kMlpIn])
+                                             const MlpPacked* P, float (&f)[kMlpIn])
 {
     const double* dr = st->dr;
     const double* m = st->drot;
@@ -206,7 +206,6 @@ __device__ __forceinline__ void mlp_features(const bt_step* __restrict__ st,
     double dtc = 0.0;
     if (r.fail || !conic_start<K>(surf, x, y, z, vx, vy, vz, dtc)) {
         // keep this lane's share of the wave's evaluation finite
-        const MlpPacked* P = static_cast<const MlpPacked*>(st->mlp);
         for (int k = 0; k < kMlpIn; k++) f[k] = P->norm[k];
         return;
     }
@@ -226,6 +225,9 @@ __device__ __forceinline__ void mlp_features(const bt_step* __restrict__ st,
 // Newton-heavy optics), mode 2 to three (left alone the compiler would
 // settle for two); the all-kinds kernel of mode 1 keeps the compiler's
 // three, where four measured slower on the TMA.
+// Networks a plan kernel keeps in LDS (45 KB per workgroup).
+constexpr int kMlpLds = 4;
+
 template <int Mode, int K>
 constexpr int plan_waves_per_simd()
 {
@@ -242,6 +244,25 @@ void k_plan(
 {
     constexpr bool Fast = Mode > 0;
     constexpr bool Mlp = Mode > 1;
+    // Mode 2: the networks of the plan's first kMlpLds MLP steps, copied to
+    // LDS once per workgroup (a wave-wide inference then reads its weight
+    // fragments from LDS, not from L2); later MLP steps run as CONIC.
+    const MlpPacked* nets = nullptr;
+    if constexpr (Mlp) {
+        __shared__ MlpPacked lds_nets[kMlpLds];
+        int slot = 0;
+        for (int k = 0; k < plan.nstep && slot < kMlpLds; k++) {
+            const bt_step* st = plan.steps + k;
+            if (st->start != BT_START_MLP || !st->mlp) continue;
+            const uint4* src = static_cast<const uint4*>(st->mlp);
+            uint4* dst = reinterpret_cast<uint4*>(lds_nets + slot);
+            for (int q = threadIdx.x; q < (int)(sizeof(MlpPacked) / 16); q += kBlock)
+                dst[q] = src[q];
+            slot++;
+        }
+        __syncthreads();
+        nets = lds_nets;
+    }
     const int64_t i0 = ray_index();
     if (!Mlp && i0 >= in.n) return;
     const bool live = i0 < in.n;
@@ -263,6 +284,7 @@ void k_plan(
     double ncm = 0.0;
     if (need_w && cm >= 0 && !r.fail) ncm = medium_n(plan.media + cm, in.wavelength[i]);
     const int nstep = plan.nstep;
+    int mslot = 0;                             // next LDS network (wave-uniform)
     for (int k = 0; k < nstep; k++) {
         if (!Mlp && r.fail && r.vig) break;    // nothing below can change this ray
         const bt_step* __restrict__ st = plan.steps + k;
@@ -277,11 +299,14 @@ void k_plan(
         }
         double correction = 0.0;
         int start = Fast ? st->start : BT_START_REFERENCE;
-        if (start == BT_START_MLP && (!Mlp || !st->mlp)) start = BT_START_CONIC;
+        if (start == BT_START_MLP && (!Mlp || !st->mlp || mslot >= kMlpLds))
+            start = BT_START_CONIC;
         if (Mlp && start == BT_START_MLP) {     // uniform: whole wave
+            const MlpPacked* P = nets + mslot;
+            mslot++;
             float f[kMlpIn];
-            mlp_features<K>(st, surf, r, f);
-            correction = (double)mlp_predict_wave(static_cast<const MlpPacked*>(st->mlp), f);
+            mlp_features<K>(st, surf, r, P, f);
+            correction = (double)mlp_predict_wave(P, f);
         }
         apply_interaction<K>(action, st->dr, st->drot, surf, n2, r, niter, start, correction);
         const int olen = st->obsc_len;
