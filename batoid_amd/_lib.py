@@ -114,11 +114,12 @@ PLAN_FAST_START = 2
 PLAN_MLP_START = 4
 PLAN_ANALYTIC = 8
 PLAN_SUM_APB = 16
+PLAN_FLAT_SUMS = 32
 ANALYTIC_KINDS = (PLANE, SPHERE, PARABOLOID, QUADRIC, ASPHERE, TILTED)
 PSF_FP32, PSF_BF16X3 = 0, 1
 GEN_POINTS, GEN_GRID, GEN_GRID_RANDOM, GEN_POLAR, GEN_POLAR_RANDOM = range(5)
 OBSC_MAX_STACK = 32
-MAX_SUM_CHILDREN = 8
+MAX_SUM_DEPTH = 3
 
 # name -> (restype, argtypes)
 _V = ctypes.c_void_p

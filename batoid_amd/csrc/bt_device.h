@@ -301,16 +301,24 @@ BT_DEV void table_sag_normal(const bt_surface* s, double x, double y, double& sa
 //    Polynomial, Bicubic), in that order (a perturbed conic mirror: figure,
 //    low-order Zernike error, measured high-frequency map).  The Sum's three
 //    children are evaluated by their own functions with no kind switch, so
-//    each Newton step carries only the registers those three need.
+//    each Newton step carries only the registers those three need;
+//  * kKindsFlatSums: every kind, Sums whose children are all leaves (the
+//    evaluation of nested Sums is left out: it costs the one-level Sum path
+//    registers).
+// kKindsAll also evaluates Sums of Sums, to kMaxSumDepth levels below a Sum.
 // Under a restricted set any other kind (or Sum layout) evaluates to NaN and
 // so fails.
 // ---------------------------------------------------------------------------
 constexpr int kKindsAll = 0;
 constexpr int kKindsAnalytic = 1;
 constexpr int kKindsSumAPB = 2;
+constexpr int kKindsFlatSums = 3;
 
 // The kind set of leaves (records that are not Sums) under set K.
-constexpr int leaf_set(int K) { return K == kKindsAll ? kKindsAll : kKindsAnalytic; }
+constexpr int leaf_set(int K)
+{
+    return (K == kKindsAll || K == kKindsFlatSums) ? kKindsAll : kKindsAnalytic;
+}
 
 // A Sum record laid out as kKindsSumAPB requires (records s+1..s+3 follow it).
 BT_DEV bool sum_is_apb(const bt_surface* s)
@@ -418,6 +426,103 @@ BT_DEV void leaf_normal(const bt_surface* s, double x, double y,
     nx = __builtin_nan(""); ny = nx; nz = nx;
 }
 
+// Sag and normal at one point (the reference evaluates them separately; the
+// results here are bit-identical, with shared work evaluated once).
+template <int K = kKindsAll>
+BT_DEV void leaf_sag_normal(const bt_surface* s, double x, double y, double& sag,
+                            double& nx, double& ny, double& nz)
+{
+    if (K == kKindsAll && s->kind == BT_BICUBIC) {
+        table_sag_normal(s, x, y, sag, nx, ny, nz);
+        return;
+    }
+    sag = leaf_sag<K>(s, x, y);
+    leaf_normal<K>(s, x, y, nx, ny, nz);
+}
+
+BT_DEV void add_slope(double& sx, double& sy, double nx, double ny, double nz)
+{
+    sx += nx/nz;
+    sy += ny/nz;
+}
+
+BT_DEV void sum_normal_from_slopes(double sx, double sy, double& nx, double& ny, double& nz)
+{
+    nz = 1./sqrt(sx*sx + sy*sy + 1);
+    nx = sx*nz;
+    ny = sy*nz;
+}
+
+// Sums of any kinds (kKindsAll).  A Sum record is followed by its children,
+// each a leaf record or a Sum with its own subtree; a Sum's n1 counts the
+// records below it, so the next sibling starts 1 + n1 records on.  Children
+// that are Sums are evaluated by the same rules (sum.cpp:23-45 calls the
+// child's own sag / normal), to kMaxSumDepth levels of nesting below a Sum
+// (the host refuses deeper trees; a deeper child evaluates to NaN).
+constexpr int kMaxSumDepth = 3;
+
+BT_DEV const bt_surface* next_child(const bt_surface* c)
+{
+    return c + (c->kind == BT_SUM ? 1 + c->n1 : 1);
+}
+
+template <int D>
+BT_DEV double sum_sag_all(const bt_surface* s, double x, double y)
+{                                                    // sum.cpp:23-28
+    double result = 0.0;
+    const bt_surface* c = s + 1;
+    const int nc = s->nchild;
+    for (int i = 0; i < nc; i++, c = next_child(c)) {
+        if (c->kind != BT_SUM) {
+            result += leaf_sag<kKindsAll>(c, x, y);
+        } else {
+            if constexpr (D > 0) result += sum_sag_all<D - 1>(c, x, y);
+            else result += __builtin_nan("");
+        }
+    }
+    return result;
+}
+
+template <int D>
+BT_DEV void sum_normal_all(const bt_surface* s, double x, double y,
+                           double& nx, double& ny, double& nz)
+{                                                    // sum.cpp:30-45
+    double sx = 0.0, sy = 0.0, tnx, tny, tnz;
+    const bt_surface* c = s + 1;
+    const int nc = s->nchild;
+    for (int i = 0; i < nc; i++, c = next_child(c)) {
+        if (c->kind != BT_SUM) {
+            leaf_normal<kKindsAll>(c, x, y, tnx, tny, tnz);
+        } else {
+            if constexpr (D > 0) sum_normal_all<D - 1>(c, x, y, tnx, tny, tnz);
+            else { tnx = __builtin_nan(""); tny = tnx; tnz = tnx; }
+        }
+        add_slope(sx, sy, tnx, tny, tnz);
+    }
+    sum_normal_from_slopes(sx, sy, nx, ny, nz);
+}
+
+template <int D>
+BT_DEV void sum_sag_normal_all(const bt_surface* s, double x, double y, double& sag,
+                               double& nx, double& ny, double& nz)
+{
+    double result = 0.0, sx = 0.0, sy = 0.0, ts, tnx, tny, tnz;
+    const bt_surface* c = s + 1;
+    const int nc = s->nchild;
+    for (int i = 0; i < nc; i++, c = next_child(c)) {
+        if (c->kind != BT_SUM) {
+            leaf_sag_normal<kKindsAll>(c, x, y, ts, tnx, tny, tnz);
+        } else {
+            if constexpr (D > 0) sum_sag_normal_all<D - 1>(c, x, y, ts, tnx, tny, tnz);
+            else { ts = __builtin_nan(""); tnx = ts; tny = ts; tnz = ts; }
+        }
+        result += ts;
+        add_slope(sx, sy, tnx, tny, tnz);
+    }
+    sag = result;
+    sum_normal_from_slopes(sx, sy, nx, ny, nz);
+}
+
 // Sum records are followed by their children (sum.cpp:23-45).  The Sum's
 // sag is 0 + child sags in child order, its normal from the sum of the
 // children's slopes (-n_x/n_z, -n_y/n_z) in child order, so the unrolled
@@ -434,24 +539,14 @@ BT_DEV double surf_sag(const bt_surface* s, double x, double y)
         result += poly_sag(s + 2, x, y);
         result += table_eval(s + 3, x, y);
         return result;
+    } else if constexpr (K == kKindsFlatSums) {
+        const int nc = s->nchild;
+        for (int i = 0; i < nc; i++)
+            result += leaf_sag<L>(s + 1 + i, x, y);
+        return result;
+    } else {
+        return sum_sag_all<kMaxSumDepth>(s, x, y);
     }
-    const int nc = s->nchild;
-    for (int i = 0; i < nc; i++)
-        result += leaf_sag<L>(s + 1 + i, x, y);
-    return result;
-}
-
-BT_DEV void add_slope(double& sx, double& sy, double nx, double ny, double nz)
-{
-    sx += nx/nz;
-    sy += ny/nz;
-}
-
-BT_DEV void sum_normal_from_slopes(double sx, double sy, double& nx, double& ny, double& nz)
-{
-    nz = 1./sqrt(sx*sx + sy*sy + 1);
-    nx = sx*nz;
-    ny = sy*nz;
 }
 
 template <int K = kKindsAll>
@@ -472,27 +567,16 @@ BT_DEV void surf_normal(const bt_surface* s, double x, double y,
         add_slope(sx, sy, tnx, tny, tnz);
         sum_normal_from_slopes(sx, sy, nx, ny, nz);
         return;
+    } else if constexpr (K == kKindsFlatSums) {
+        const int nc = s->nchild;
+        for (int i = 0; i < nc; i++) {
+            leaf_normal<L>(s + 1 + i, x, y, tnx, tny, tnz);
+            add_slope(sx, sy, tnx, tny, tnz);
+        }
+        sum_normal_from_slopes(sx, sy, nx, ny, nz);
+    } else {
+        sum_normal_all<kMaxSumDepth>(s, x, y, nx, ny, nz);
     }
-    const int nc = s->nchild;
-    for (int i = 0; i < nc; i++) {
-        leaf_normal<L>(s + 1 + i, x, y, tnx, tny, tnz);
-        add_slope(sx, sy, tnx, tny, tnz);
-    }
-    sum_normal_from_slopes(sx, sy, nx, ny, nz);
-}
-
-// Sag and normal at one point (the reference evaluates them separately; the
-// results here are bit-identical, with shared work evaluated once).
-template <int K = kKindsAll>
-BT_DEV void leaf_sag_normal(const bt_surface* s, double x, double y, double& sag,
-                            double& nx, double& ny, double& nz)
-{
-    if (K == kKindsAll && s->kind == BT_BICUBIC) {
-        table_sag_normal(s, x, y, sag, nx, ny, nz);
-        return;
-    }
-    sag = leaf_sag<K>(s, x, y);
-    leaf_normal<K>(s, x, y, nx, ny, nz);
 }
 
 template <int K = kKindsAll>
@@ -520,15 +604,18 @@ BT_DEV void surf_sag_normal(const bt_surface* s, double x, double y, double& sag
         sag = result;
         sum_normal_from_slopes(sx, sy, nx, ny, nz);
         return;
+    } else if constexpr (K == kKindsFlatSums) {
+        const int nc = s->nchild;
+        for (int i = 0; i < nc; i++) {
+            leaf_sag_normal<L>(s + 1 + i, x, y, ts, tnx, tny, tnz);
+            result += ts;
+            add_slope(sx, sy, tnx, tny, tnz);
+        }
+        sag = result;
+        sum_normal_from_slopes(sx, sy, nx, ny, nz);
+    } else {
+        sum_sag_normal_all<kMaxSumDepth>(s, x, y, sag, nx, ny, nz);
     }
-    const int nc = s->nchild;
-    for (int i = 0; i < nc; i++) {
-        leaf_sag_normal<L>(s + 1 + i, x, y, ts, tnx, tny, tnz);
-        result += ts;
-        add_slope(sx, sy, tnx, tny, tnz);
-    }
-    sag = result;
-    sum_normal_from_slopes(sx, sy, nx, ny, nz);
 }
 
 // Surface evaluators for the Newton loops.  SurfEval<K> dispatches on the
@@ -552,6 +639,25 @@ struct SurfEval {
                                   double& nx, double& ny, double& nz)
     {
         surf_sag_normal<K>(s, x, y, sag, nx, ny, nz);
+    }
+};
+
+// A record known to be a leaf (not a Sum).
+template <int K>
+struct LeafEval {
+    static BT_DEV double sag(const bt_surface* s, double x, double y)
+    {
+        return leaf_sag<K>(s, x, y);
+    }
+    static BT_DEV void normal(const bt_surface* s, double x, double y, double& nx, double& ny,
+                              double& nz)
+    {
+        leaf_normal<K>(s, x, y, nx, ny, nz);
+    }
+    static BT_DEV void sag_normal(const bt_surface* s, double x, double y, double& sag,
+                                  double& nx, double& ny, double& nz)
+    {
+        leaf_sag_normal<K>(s, x, y, sag, nx, ny, nz);
     }
 };
 
@@ -764,10 +870,28 @@ BT_DEV bool leaf_tti(const bt_surface* s, double x, double y, double z,
     case BT_BICUBIC:
     case BT_POLYNOMIAL:                              // surface.cpp:22-49
         if constexpr (K == kKindsAll)
-            return newton<SurfEval<K>>(s, x, y, z, vx, vy, vz, dt, niter, memo);
+            return newton<LeafEval<K>>(s, x, y, z, vx, vy, vz, dt, niter, memo);
         break;
     }
     return false;
+}
+
+// sum.cpp:47-56: the first child's intersection (itself a Sum's, to the
+// nesting depth kKindsAll allows) seeds Newton on the Sum.
+template <int D>
+BT_DEV bool sum_tti_all(const bt_surface* s, double x, double y, double z,
+                        double vx, double vy, double vz, double& dt, int niter,
+                        NormalMemo* memo)
+{
+    const bt_surface* c = s + 1;
+    bool ok = false;
+    if (c->kind != BT_SUM) {
+        ok = leaf_tti<kKindsAll>(c, x, y, z, vx, vy, vz, dt, niter);
+    } else {
+        if constexpr (D > 0) ok = sum_tti_all<D - 1>(c, x, y, z, vx, vy, vz, dt, niter, nullptr);
+    }
+    if (!ok) return false;
+    return newton<SurfEval<kKindsAll>>(s, x, y, z, vx, vy, vz, dt, niter, memo);
 }
 
 template <int K = kKindsAll>
@@ -778,13 +902,16 @@ BT_DEV bool surf_tti(const bt_surface* s, double x, double y, double z,
     constexpr int L = leaf_set(K);
     if (K == kKindsAnalytic || s->kind != BT_SUM)
         return leaf_tti<L>(s, x, y, z, vx, vy, vz, dt, niter, memo);
-    if (K == kKindsSumAPB && !sum_is_apb(s)) return false;
-    // sum.cpp:47-56: first child's intersection seeds Newton on the Sum
-    if (!leaf_tti<L>(s + 1, x, y, z, vx, vy, vz, dt, niter)) return false;
-    if constexpr (K == kKindsSumAPB)
+    if constexpr (K == kKindsSumAPB) {
+        if (!sum_is_apb(s)) return false;
+        if (!leaf_tti<L>(s + 1, x, y, z, vx, vy, vz, dt, niter)) return false;
         return newton<SumAPBEval>(s, x, y, z, vx, vy, vz, dt, niter, memo);
-    else
+    } else if constexpr (K == kKindsFlatSums) {
+        if (!leaf_tti<L>(s + 1, x, y, z, vx, vy, vz, dt, niter)) return false;
         return newton<SurfEval<K>>(s, x, y, z, vx, vy, vz, dt, niter, memo);
+    } else {
+        return sum_tti_all<kMaxSumDepth>(s, x, y, z, vx, vy, vz, dt, niter, memo);
+    }
 }
 
 // The surface's analytic base start: the root of a Sum's first child or of an
@@ -794,7 +921,9 @@ template <int K = kKindsAll>
 BT_DEV bool conic_start(const bt_surface* s, double x, double y, double z, double vx,
                         double vy, double vz, double& dt)
 {
-    const bt_surface* c = (K != kKindsAnalytic && s->kind == BT_SUM) ? s + 1 : s;
+    const bt_surface* c = s;
+    if (K != kKindsAnalytic)
+        while (c->kind == BT_SUM) c = c + 1;     // the first leaf of nested Sums
     dt = 0.0;
     switch (c->kind) {
     case BT_ASPHERE:

@@ -233,7 +233,7 @@ constexpr int plan_waves_per_simd()
 {
     if (K == kKindsAnalytic && Mode < 2) return 5;
     if (K == kKindsSumAPB) return Mode == 2 ? 3 : 4;
-    return Mode == 0 ? 4 : (Mode == 2 ? 3 : 1);
+    return Mode == 0 ? 4 : (Mode == 2 ? 3 : 1);     // kKindsAll, kKindsFlatSums
 }
 
 template <int Mode, int K>
@@ -605,22 +605,26 @@ int bt_trace_plan(bt_plan plan, bt_rays in, bt_rays out, int niter, void* stream
     if (in.n == 0) return 0;
     const int need_w = (plan.flags & BT_PLAN_NO_REFRACT) ? 0 : 1;
     const int kinds = (plan.flags & BT_PLAN_ANALYTIC) ? kKindsAnalytic
-                      : (plan.flags & BT_PLAN_SUM_APB) ? kKindsSumAPB : kKindsAll;
+                      : (plan.flags & BT_PLAN_SUM_APB) ? kKindsSumAPB
+                      : (plan.flags & BT_PLAN_FLAT_SUMS) ? kKindsFlatSums : kKindsAll;
     const int mode = (plan.flags & BT_PLAN_MLP_START) ? 2 : (plan.flags & BT_PLAN_FAST_START) ? 1 : 0;
     const dim3 grid(grid_for(in.n)), block(kBlock);
     hipStream_t st = as_stream(stream);
 #define BT_PLAN_LAUNCH(M, K) \
     hipLaunchKernelGGL((k_plan<M, K>), grid, block, 0, st, plan, in, out, niter, need_w)
-    switch (mode * 3 + kinds) {
+    switch (mode * 4 + kinds) {
     case 0: BT_PLAN_LAUNCH(0, kKindsAll); break;
     case 1: BT_PLAN_LAUNCH(0, kKindsAnalytic); break;
     case 2: BT_PLAN_LAUNCH(0, kKindsSumAPB); break;
-    case 3: BT_PLAN_LAUNCH(1, kKindsAll); break;
-    case 4: BT_PLAN_LAUNCH(1, kKindsAnalytic); break;
-    case 5: BT_PLAN_LAUNCH(1, kKindsSumAPB); break;
-    case 6: BT_PLAN_LAUNCH(2, kKindsAll); break;
-    case 7: BT_PLAN_LAUNCH(2, kKindsAnalytic); break;
-    default: BT_PLAN_LAUNCH(2, kKindsSumAPB); break;
+    case 3: BT_PLAN_LAUNCH(0, kKindsFlatSums); break;
+    case 4: BT_PLAN_LAUNCH(1, kKindsAll); break;
+    case 5: BT_PLAN_LAUNCH(1, kKindsAnalytic); break;
+    case 6: BT_PLAN_LAUNCH(1, kKindsSumAPB); break;
+    case 7: BT_PLAN_LAUNCH(1, kKindsFlatSums); break;
+    case 8: BT_PLAN_LAUNCH(2, kKindsAll); break;
+    case 9: BT_PLAN_LAUNCH(2, kKindsAnalytic); break;
+    case 10: BT_PLAN_LAUNCH(2, kKindsSumAPB); break;
+    default: BT_PLAN_LAUNCH(2, kKindsFlatSums); break;
     }
 #undef BT_PLAN_LAUNCH
     return check_launch("bt_trace_plan");

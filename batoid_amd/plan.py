@@ -79,6 +79,23 @@ def _sums_are_apb(recs):
     return True
 
 
+def _has_nested_sum(recs):
+    """True when a Sum record has a Sum among its children."""
+    i = 0
+    while i < len(recs):
+        r = recs[i]
+        if r.kind == _lib.SUM:
+            c = i + 1
+            for _ in range(r.nchild):
+                if recs[c].kind == _lib.SUM:
+                    return True
+                c += 1
+            i = c
+        else:
+            i += 1
+    return False
+
+
 class PlanSpec:
     """Host-side plan: ctypes records plus the numpy tables they point to."""
 
@@ -149,6 +166,9 @@ class PlanSpec:
                       | (_lib.PLAN_ANALYTIC if analytic else 0)
                       | (_lib.PLAN_SUM_APB if not analytic and _sums_are_apb(self.surf_recs)
                          else 0))
+        if not self.flags & (_lib.PLAN_ANALYTIC | _lib.PLAN_SUM_APB) \
+                and not _has_nested_sum(self.surf_recs):
+            self.flags |= _lib.PLAN_FLAT_SUMS
 
 
 class DevicePlan:

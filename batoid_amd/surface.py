@@ -358,8 +358,9 @@ class Bicubic(Surface):
 
 class Sum(Surface):
     """Sum of surfaces; intersection seeds Newton with the first surface's
-    intersection (surface.py:666-710, sum.cpp).  Children must not
-    themselves be Sums."""
+    intersection (surface.py:666-710, sum.cpp).  Any number of children;
+    children may themselves be Sums, nested at most MAX_SUM_DEPTH levels
+    below this one (the GPU evaluates nested Sums to that depth)."""
 
     def __init__(self, *args):
         from collections.abc import Sequence
@@ -368,18 +369,23 @@ class Sum(Surface):
         assert all(isinstance(arg, Surface) for arg in args)
         self.surfaces = tuple(args)
 
+    def _depth(self):
+        return 1 + max((s._depth() for s in self.surfaces if isinstance(s, Sum)), default=0)
+
     def _records(self):
-        if len(self.surfaces) > _lib.MAX_SUM_CHILDREN:
-            raise ValueError(f"Sum of more than {_lib.MAX_SUM_CHILDREN} surfaces")
-        recs = [_rec(_lib.SUM, nchild=len(self.surfaces))]
-        arrs = [(None,) * 4]
+        """The Sum record, then each child's records in order; n1 counts the
+        records below this one, so a child Sum's subtree can be stepped
+        over."""
+        if self._depth() > 1 + _lib.MAX_SUM_DEPTH:
+            raise NotImplementedError(
+                f"Sums nested more than {_lib.MAX_SUM_DEPTH} levels deep")
+        recs, arrs = [], []
         for s in self.surfaces:
-            if isinstance(s, Sum):
-                raise NotImplementedError("nested Sum surfaces are not supported")
-            r, a = s._leaf_record()
-            recs.append(r)
-            arrs.append(a)
-        return recs, arrs
+            r, a = s._records()
+            recs += r
+            arrs += a
+        head = _rec(_lib.SUM, nchild=len(self.surfaces), n1=len(recs))
+        return [head] + recs, [(None,) * 4] + arrs
 
     def __hash__(self):
         return hash(("batoid.Sum", tuple(self.surfaces)))

@@ -55,8 +55,11 @@ typedef struct bt_rays {
 } bt_rays;
 
 /* ---- surfaces -----------------------------------------------------------
- * One record per surface; a Sum record is immediately followed by its
- * `nchild` child records (children may not be Sums).  Derived constants are
+ * One record per surface; a Sum record is immediately followed by the
+ * records of its `nchild` children in order, each a leaf record or a Sum
+ * with its own children (nested at most 3 levels below a Sum), and its n1
+ * is the number of records below it (so a child Sum's subtree can be
+ * stepped over).  Derived constants are
  * filled by the host exactly as the reference constructors compute them.
  *   PLANE        plane.cpp:13-30
  *   SPHERE       p = {R, R*R, 1/R, 1/R/R}                 sphere.cpp:9-72
@@ -72,14 +75,13 @@ typedef struct bt_rays {
  *   POLYNOMIAL   p = {xorigin, yorigin}; n1 = xsize, n2 = ysize;
  *                a0 = coefs[ysize][xsize], a1 = gradx, a2 = grady
  *                ([ysize-1][xsize-1])                     polynomialSurface.cpp:35-66
- *   SUM          nchild children follow                   sum.cpp:23-56
+ *   SUM          nchild children follow; n1 = records below   sum.cpp:23-56
  */
 enum bt_surface_kind {
     BT_PLANE = 0, BT_SPHERE = 1, BT_PARABOLOID = 2, BT_QUADRIC = 3,
     BT_ASPHERE = 4, BT_TILTED = 5, BT_BICUBIC = 6, BT_POLYNOMIAL = 7,
     BT_SUM = 8
 };
-#define BT_MAX_SUM_CHILDREN 8
 
 typedef struct bt_surface {
     int32_t kind;
@@ -197,6 +199,11 @@ typedef struct bt_step {
                                   Bicubic, polynomial and Sum code (fewer
                                   registers); under this flag any other kind
                                   evaluates to NaN and fails              */
+#define BT_PLAN_FLAT_SUMS 32   /* (without the two flags above) no Sum has
+                                  a Sum child: the trace runs a kernel
+                                  without the nested-Sum evaluation (fewer
+                                  registers); under this flag a Sum child
+                                  of a Sum evaluates to NaN and fails      */
 #define BT_PLAN_SUM_APB   16   /* (without BT_PLAN_ANALYTIC) every surface
                                   record is one of the kinds above or a Sum
                                   of exactly (Asphere, Polynomial, Bicubic)

@@ -163,6 +163,14 @@ static double asphere_dzdr(const bt_surface* s, double r)
     return result;
 }
 
+/* Records of a surface: a Sum is followed by its children, each a leaf
+ * record or a Sum with its own children; a Sum's n1 counts all records
+ * below it, so its next sibling starts 1 + n1 records on. */
+static int subtree_len(const bt_surface* s)
+{
+    return s->kind == BT_SUM ? 1 + s->n1 : 1;
+}
+
 double or_sag(const bt_surface* s, double x, double y)
 {
     switch (s->kind) {
@@ -199,8 +207,9 @@ double or_sag(const bt_surface* s, double x, double y)
         return horner2d(x-s->p[0], y-s->p[1], s->a0, s->n1, s->n2);
     case BT_SUM: {                                   /* sum.cpp:23-28 */
         double result = 0.0;
-        for (int i = 0; i < s->nchild; i++)
-            result += or_sag(s + 1 + i, x, y);
+        const bt_surface* c = s + 1;
+        for (int i = 0; i < s->nchild; i++, c += subtree_len(c))
+            result += or_sag(c, x, y);
         return result;
     }
     }
@@ -288,9 +297,10 @@ void or_normal(const bt_surface* s, double x, double y,
     }
     case BT_SUM: {                                   /* sum.cpp:30-45 */
         double sx = 0.0, sy = 0.0;
-        for (int i = 0; i < s->nchild; i++) {
+        const bt_surface* c = s + 1;
+        for (int i = 0; i < s->nchild; i++, c += subtree_len(c)) {
             double tnx, tny, tnz;
-            or_normal(s + 1 + i, x, y, &tnx, &tny, &tnz);
+            or_normal(c, x, y, &tnx, &tny, &tnz);
             sx += tnx/tnz;
             sy += tny/tnz;
         }

@@ -68,11 +68,17 @@ const batoid::Surface* build_leaf(const bt_surface* s, Built& b)
     return out;
 }
 
+// A Sum's children follow it, each a leaf record or a Sum with its n1
+// descendant records after it.
 const batoid::Surface* build_surface(const bt_surface* s, Built& b)
 {
     if (s->kind != BT_SUM) return build_leaf(s, b);
     std::unique_ptr<const batoid::Surface*[]> kids(new const batoid::Surface*[s->nchild]);
-    for (int i = 0; i < s->nchild; i++) kids[i] = build_leaf(s + 1 + i, b);
+    const bt_surface* c = s + 1;
+    for (int i = 0; i < s->nchild; i++) {
+        kids[i] = build_surface(c, b);
+        c += (c->kind == BT_SUM) ? 1 + c->n1 : 1;
+    }
     auto* sum = new batoid::Sum(kids.get(), (size_t)s->nchild);
     b.lists.push_back(std::move(kids));
     b.surfaces.emplace_back(sum);

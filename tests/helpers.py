@@ -51,6 +51,19 @@ def surfaces():
         ("sum", bt.Sum(bt.Asphere(9.0, -0.9, [1e-4, -2e-6]), zernike_surface(seed=58),
                        bicubic_surface(seed=6))),
         ("sum_plane", bt.Sum(bt.Plane(), bt.Paraboloid(12.0))),
+        # Sums of Sums (sum.cpp evaluates each child, Sum or not, by its own
+        # sag / normal / timeToIntersect), nested to the three levels the
+        # GPU evaluates, and a Sum of eleven children
+        ("sum_nested", bt.Sum(bt.Sum(bt.Asphere(9.0, -0.9, [1e-4]), zernike_surface(seed=59)),
+                              bt.Sum(bicubic_surface(seed=7), bt.Tilted(0.001, 0.0005)),
+                              bt.Plane())),
+        ("sum_deep", bt.Sum(bt.Sum(bt.Sum(bt.Sum(bt.Paraboloid(8.0), bicubic_surface(seed=8)),
+                                          bt.Tilted(-0.002, 0.001)),
+                                   zernike_surface(seed=60, jmax=11)),
+                            bt.Sphere(40.0))),
+        ("sum_wide", bt.Sum([bt.Quadric(7.0, -0.4), bicubic_surface(seed=9)]
+                            + [zernike_surface(seed=61 + k, jmax=6) for k in range(8)]
+                            + [bt.Tilted(0.0005, -0.0007)])),
     ]
 
 

@@ -277,18 +277,59 @@ def test_plan_kernels_agree_on_sum_layouts():
     a = rv.copy()
     optic.trace(a)
     b = rv.copy()
-    st = _lib.BtPlan.from_buffer_copy(plan.struct)
-    st.flags &= ~_lib.PLAN_SUM_APB
-    _lib.call("bt_trace_plan", st, b._rays, b._rays, NITER, _device.stream())
-    assert_cols_identical(_host(b), _host(a), what="all-kinds vs APB kernel")
+    for other in (0, _lib.PLAN_FLAT_SUMS):            # nested-capable, flat Sums
+        b = rv.copy()
+        st = _lib.BtPlan.from_buffer_copy(plan.struct)
+        st.flags = (st.flags & ~_lib.PLAN_SUM_APB) | other
+        _lib.call("bt_trace_plan", st, b._rays, b._rays, NITER, _device.stream())
+        assert_cols_identical(_host(b), _host(a), what=f"kernel {other} vs APB kernel")
 
     free = workloads.freeform_tma()
     rv = workloads.tma_rays(free, thx[3], thy[3], 770e-9, nrad=60, naz=600)
-    assert not free._plan(rv.coordSys, False).struct.flags & _lib.PLAN_SUM_APB
+    fplan = free._plan(rv.coordSys, False)
+    assert fplan.struct.flags & _lib.PLAN_FLAT_SUMS
+    assert not fplan.struct.flags & _lib.PLAN_SUM_APB
     ref = HostRays.from_rv(rv)
     HostPlan(PlanSpec(collect_steps(free), rv.coordSys)).trace(ref, niter=NITER)
+    b = rv.copy()
     free.trace(rv)
     assert_cols_identical(_host(rv), ref.cols, what="freeform TMA")
+    st = _lib.BtPlan.from_buffer_copy(fplan.struct)
+    st.flags &= ~_lib.PLAN_FLAT_SUMS
+    _lib.call("bt_trace_plan", st, b._rays, b._rays, NITER, _device.stream())
+    assert_cols_identical(_host(b), ref.cols, what="freeform TMA, nested-capable kernel")
+
+
+def test_nested_sum_plan_bit_exact():
+    """Mirrors whose surfaces are Sums of Sums (three levels) and a Sum of
+    eleven children, traced by the fused plan (the nested-capable kernel):
+    bit-identical to the oracle, which equals the reference's compiled core
+    on these surfaces (test_oracle_vs_ref).  With the flat-Sum flag forced
+    on, the nested records are not read as leaves' children: such rays fail."""
+    from batoid_amd import _device
+    s = dict(surfaces())
+    cs = bt.CoordSys()
+    for name in ("sum_nested", "sum_deep", "sum_wide"):
+        optic = bt.CompoundOptic([
+            bt.Mirror(s[name], name="M", coordSys=cs,
+                      obscuration=bt.ObscNegation(bt.ObscCircle(1.0))),
+            bt.Detector(bt.Plane(), name="D", coordSys=cs.shiftLocal([0, 0, 2.5]))],
+            name=name, inMedium=bt.vacuum)
+        base = HostRays(**random_rays(N, seed=31, z0=2.0, spread=0.05, radius=1.1))
+        spec = PlanSpec(collect_steps(optic), bt.CoordSys())
+        nested = name != "sum_wide"
+        assert bool(spec.flags & _lib.PLAN_FLAT_SUMS) == (not nested)
+        ref = HostPlan(spec).trace(base.copy(), niter=NITER)
+        assert 0 < ref["vignetted"].sum() < N
+        rv = _gpu_rays(base)
+        optic.trace(rv)
+        assert_cols_identical(_host(rv), ref.cols, what=f"plan {name}")
+        if nested:
+            rv = _gpu_rays(base)
+            st = _lib.BtPlan.from_buffer_copy(optic._plan(rv.coordSys, False).struct)
+            st.flags |= _lib.PLAN_FLAT_SUMS
+            _lib.call("bt_trace_plan", st, rv._rays, rv._rays, NITER, _device.stream())
+            assert rv.failed[~base["failed"].astype(bool)].all()
 
 
 def test_sum_layout_flag_on_other_sums_fails_rays():

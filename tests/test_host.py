@@ -142,7 +142,7 @@ def test_yaml_and_plan():
     assert PlanSpec(collect_steps(pert), bt.CoordSys()).flags == _lib.PLAN_SUM_APB
     # other Sum layouts (and Bicubic or polynomial leaves) need the all-kinds kernel
     free = workloads.freeform_tma()
-    assert PlanSpec(collect_steps(free), bt.CoordSys()).flags == 0
+    assert PlanSpec(collect_steps(free), bt.CoordSys()).flags == _lib.PLAN_FLAT_SUMS
     from batoid_amd.plan import _sums_are_apb
     a, p, b = pert['M1'].surface.surfaces
     for surf, want in ((bt.Sum(a, p, b), True), (bt.Sum(a, b, p), False),
@@ -265,3 +265,28 @@ def test_insert_and_remove_optic():
     assert rem == telescope
     with pytest.raises(ValueError):
         telescope.withRemovedOptic("noSuchItem")
+
+
+def test_nested_sum_records():
+    """A Sum's records: the Sum, then each child's records in order; n1
+    counts the records below it.  Sums nest to MAX_SUM_DEPTH levels below a
+    Sum and take any number of children."""
+    a, b, c, d = bt.Plane(), bt.Sphere(3.0), bt.Paraboloid(5.0), bt.Tilted(0.01, 0.0)
+    inner = bt.Sum(b, c)
+    recs, arrs = bt.Sum(a, inner, d)._records()
+    assert [r.kind for r in recs] == [_lib.SUM, _lib.PLANE, _lib.SUM, _lib.SPHERE,
+                                      _lib.PARABOLOID, _lib.TILTED]
+    assert (recs[0].nchild, recs[0].n1) == (3, 5)
+    assert (recs[2].nchild, recs[2].n1) == (2, 2)
+    assert len(arrs) == len(recs)
+    wide = bt.Sum([bt.Plane()] * 20)
+    assert wide._records()[0][0].nchild == 20
+    s = a
+    for _ in range(_lib.MAX_SUM_DEPTH + 1):
+        s = bt.Sum(s, d)
+    s._records()                                   # 3 levels below the top: fine
+    with pytest.raises(NotImplementedError):
+        bt.Sum(s, d)._records()
+    # plan flags: nested Sums need the nested-capable kernel
+    from batoid_amd.plan import _has_nested_sum
+    assert _has_nested_sum(recs) and not _has_nested_sum(bt.Sum(a, b)._records()[0])
