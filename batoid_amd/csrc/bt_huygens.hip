@@ -226,10 +226,20 @@ __device__ __forceinline__ float step_phase(float c, float hi, float lo, float a
 // (split-BF16) over the 128 x 128 tile, and the waves per SIMD each is
 // compiled for: the FP32 wave's 256 accumulators need the whole register
 // file (one wave per SIMD), split-BF16 runs two.
+// Column blocks of 32 per wave in split-BF16: 4 (64 x 128 wave tiles, two
+// waves per workgroup, one per SIMD) or 2 (64 x 64, four waves, two per
+// SIMD).  The wider tile spends the row operands' generation on twice as
+// many MFMAs.
+#ifndef BT_BF16_CB
+#define BT_BF16_CB 4
+#endif
+constexpr int kBf16Cb = BT_BF16_CB;
 template <int kPrec>
-constexpr int gemm_threads() { return kPrec == BT_PSF_FP32 ? 128 : 256; }
+constexpr int gemm_cb() { return kPrec == BT_PSF_FP32 ? 4 : kBf16Cb; }
 template <int kPrec>
-constexpr int gemm_waves_per_simd() { return kPrec == BT_PSF_FP32 ? 1 : 2; }
+constexpr int gemm_threads() { return gemm_cb<kPrec>() == 4 ? 128 : 256; }
+template <int kPrec>
+constexpr int gemm_waves_per_simd() { return gemm_cb<kPrec>() == 4 ? 1 : 2; }
 
 // One workgroup's 128 x 128 complex tile of the sum over rays [r0, r1) of
 // the prepared parameters, written as FP32 (re, im) into out[i][j][2] (the
@@ -240,12 +250,12 @@ __device__ __forceinline__ void psf_tile(const float* __restrict__ params,
                                          int tile_i0, int tile_j0, int nx, int ny, int nyp,
                                          float* __restrict__ out)
 {
-    // column blocks of 32 per wave: 4 (FP32, 64 x 128) or 2 (split-BF16, 64 x 64)
-    constexpr int CB = kPrec == BT_PSF_FP32 ? 4 : 2;
+    // column blocks of 32 per wave: 4 (64 x 128) or 2 (split-BF16 64 x 64)
+    constexpr int CB = gemm_cb<kPrec>();
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int i0 = tile_i0 + (wave & 1) * 64;
-    const int j0 = tile_j0 + (wave >> 1) * 64;
+    const int j0 = tile_j0 + (wave >> 1) * (32 * CB);
     const int half = lane >> 5;       // k half of the operand
     const int l32 = lane & 31;
     v16f acc[4][CB];
@@ -394,15 +404,36 @@ __device__ __forceinline__ void psf_tile(const float* __restrict__ params,
                 rc[q] = __builtin_amdgcn_raw_buffer_load_b128(rq, k * kRotStride * 4, 0, 0);
                 rd[q] = __builtin_amdgcn_raw_buffer_load_b64(rq, k * kRotStride * 4 + 16, 0, 0);
             }
-            // column block 0 evaluated, block 1 (32 pixels on) rotated from it
-            uint32_t bhi[2][4], blo[2][4];                    // [block][ray q]
+            // column block 0 evaluated, blocks 1 .. CB-1 (32 k pixels on)
+            // rotated from it
+            v4u cc[4];
+            v2u cd[4];
+            if constexpr (CB > 2) {
+                for (int q = 0; q < 4; q++) {
+                    const int k = 4*half + q;
+                    cc[q] = __builtin_amdgcn_raw_buffer_load_b128(rq, k * kRotStride * 4 + 32,
+                                                                  0, 0);
+                    cd[q] = __builtin_amdgcn_raw_buffer_load_b64(rq, k * kRotStride * 4 + 48,
+                                                                 0, 0);
+                }
+            }
+            uint32_t bhi[CB][4], blo[CB][4];                  // [block][ray q]
             for (int q = 0; q < 4; q++) {
                 const float thb = step_phase(cj[0], F(g.p[q].z), F(g.p[q].w), 0.f);
                 const float c = cos_rev(thb), sn = sin_rev(thb);
                 split_pair(c, sn, bhi[0][q], blo[0][q]);
-                const float cq = F(g.f[q].z), sq = F(g.f[q].w);
-                split_pair(__builtin_fmaf(c, cq, -(sn * sq)), __builtin_fmaf(sn, cq, c * sq),
-                           bhi[1][q], blo[1][q]);
+                for (int kb = 1; kb < CB; kb++) {
+                    float cq, sq;
+                    if constexpr (CB > 2) {
+                        cq = F(kb == 3 ? cd[q].x : cc[q][2*kb - 2]);
+                        sq = F(kb == 3 ? cd[q].y : cc[q][2*kb - 1]);
+                    } else {
+                        cq = F(g.f[q].z);
+                        sq = F(g.f[q].w);
+                    }
+                    split_pair(__builtin_fmaf(c, cq, -(sn * sq)), __builtin_fmaf(sn, cq, c * sq),
+                               bhi[kb][q], blo[kb][q]);
+                }
             }
             // row block 0 evaluated, blocks 1-3 rotated from it; the lane's k
             // parts (v0, v1) of U = c + i s are (c, -s) on a real row and
@@ -428,7 +459,7 @@ __device__ __forceinline__ void psf_tile(const float* __restrict__ params,
                                    __builtin_fmaf(y[q], cr, -(x[q] * sr)), ahi[q], alo[q]);
                     }
                 }
-                for (int cb = 0; cb < 2; cb++) {
+                for (int cb = 0; cb < CB; cb++) {
                     v16f a = acc[rb][cb];
                     a = __builtin_amdgcn_mfma_f32_32x32x16_bf16(op(alo), op(bhi[cb]), a, 0, 0, 0);
                     a = __builtin_amdgcn_mfma_f32_32x32x16_bf16(op(ahi), op(blo[cb]), a, 0, 0, 0);

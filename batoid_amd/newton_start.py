@@ -73,6 +73,7 @@ class NewtonStartNet:
         self.out_mean = 0.0
         self.out_std = 1.0
         self.history = []
+        self.samples = 0
         self._packed = None
 
     # -- normalisation ------------------------------------------------------
@@ -119,6 +120,7 @@ class NewtonStartNet:
         n = features.shape[0]
         if n == 0:
             raise ValueError("no training samples")
+        self.samples = n
         self.set_normalisation(features, delta)
         targets = ((delta - self.out_mean) / self.out_std).to(torch.float32).contiguous()
         batch = min(batch, n)
@@ -195,6 +197,7 @@ class NewtonStartNet:
         net.out_mean = float(state["out_mean"])
         net.out_std = float(state["out_std"])
         net.history = []
+        net.samples = 0
         net._packed = None
         return net
 

@@ -672,12 +672,16 @@ def bench_newton_start(nrad, naz, optic_name="perturbed_tma", reps=5, train_step
     from batoid_amd.newton_start import train_newton_starts
     optic = getattr(workloads, optic_name)()
     thx, thy = workloads.tma_fields()
+    # training set: 12 other fields x 838,554 hexapolar rays = 10,062,648
+    # tracer-generated rays (the 1e7 of the learned-start configuration)
     t0 = time.perf_counter()
-    train_rvs = [workloads.tma_rays(optic, thx[k], thy[k], 620e-9, nrad=60, naz=600)
+    train_rvs = [workloads.tma_rays(optic, thx[k], thy[k], 620e-9, nrad=325, naz=3205)
                  for k in range(6, 18)]
+    train_rays = sum(len(r) for r in train_rvs)
     nets = train_newton_starts(optic, train_rvs, steps=train_steps, batch=8192)
     torch.cuda.synchronize()
     train_s = time.perf_counter() - t0
+    del train_rvs
     log(f"newton-start networks trained in {train_s:.1f} s: "
         + ", ".join(f"{k} loss {v.history[-1][1]:.2e}" for k, v in nets.items()))
     rv0 = workloads.tma_rays(optic, thx[0], thy[0], 620e-9, nrad=nrad, naz=naz)
@@ -689,7 +693,8 @@ def bench_newton_start(nrad, naz, optic_name="perturbed_tma", reps=5, train_step
                                          "+ perturbation, Bicubic) per mirror)"}[optic_name]
                        + ", field 0, 620 nm",
            "rays": n, "interfaces": nsteps, "train_seconds": train_s,
-           "train_samples_per_surface": 12 * len(train_rvs[0]),
+           "train_rays": train_rays,
+           "train_samples_per_surface": {k: int(v.samples) for k, v in nets.items()},
            "train_steps": train_steps}
     ref_cols = None
     for mode in ("reference", "conic", "mlp"):
