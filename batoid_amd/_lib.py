@@ -152,6 +152,7 @@ SIGNATURES = {
     "bt_surface_normal": (_I, [_V, _V, _V, _V, _L, _V]),
     "bt_medium_get_n": (_I, [_V, _V, _V, _L, _V]),
     "bt_obsc_contains": (_I, [_V, _I, _V, _V, _V, _L, _V]),
+    "bt_check_sqrt": (_I, [_V, _V, _L, _V]),
     "bt_position_at_time": (_I, [BtRays, _D, _V, _V, _V, _V]),
     "bt_propagate_in_place": (_I, [BtRays, _D, _V]),
     "bt_phase": (_I, [BtRays, _D, _D, _D, _D, _V, _V]),

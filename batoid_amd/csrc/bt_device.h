@@ -22,6 +22,69 @@
 namespace bt {
 
 // ---------------------------------------------------------------------------
+// sqrt(x) and 1/sqrt(x), correctly rounded, without range handling where it
+// cannot change a bit.
+//
+// The compiler's IEEE sqrt scales x < 2^-767 by 2^256 before the v_rsq_f64
+// estimate, refines it (g = x r, h = r/2; one coupled Goldschmidt step; two
+// Newton corrections from the residual d = x - g*g), unscales, and selects x
+// itself for +-0 and +inf.  Its IEEE division rescales operands of extreme
+// exponent (v_div_scale), refines v_rcp_f64 by two Newton steps, applies one
+// Markstein correction q + (n - d q) r, and patches special operands
+// (v_div_fixup).  For x in [2^-767, DBL_MAX] all of that range handling is
+// the identity: the scale factors are 2^0 and nothing is special, and the
+// root s lies in [2^-384, 2^512], where 1/s is not rescaled or patched
+// either (the quotient 1*r of the first step is r itself).  There the same
+// refinement steps run here without it; the range handling is 8 of the 18
+// vector instructions of a sqrt and 12 of the 29 of a 1/sqrt, the range
+// test that replaces it two compares.  Every other x (zero, tiny, negative,
+// infinite, NaN) takes the compiler's full sequence, so cr_sqrt(x) and
+// cr_rsqrt(x) are bit-identical to sqrt(x) and 1/sqrt(x) for every input
+// (bt_check_sqrt exposes both for testing).
+// ---------------------------------------------------------------------------
+BT_DEV bool sqrt_in_plain_range(double x)
+{
+    return x >= 0x1p-767 && x <= 0x1.fffffffffffffp+1023;
+}
+
+BT_DEV double sqrt_plain_range(double x)
+{
+    const double r = __builtin_amdgcn_rsq(x);
+    double g = x * r;
+    double h = r * 0.5;
+    const double e = __builtin_fma(-h, g, 0.5);
+    g = __builtin_fma(g, e, g);
+    h = __builtin_fma(h, e, h);
+    double d = __builtin_fma(-g, g, x);
+    g = __builtin_fma(d, h, g);
+    d = __builtin_fma(-g, g, x);
+    return __builtin_fma(d, h, g);
+}
+
+BT_DEV double recip_plain_range(double s)
+{
+    double r = __builtin_amdgcn_rcp(s);
+    double e = __builtin_fma(-s, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    e = __builtin_fma(-s, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    const double rem = __builtin_fma(-s, r, 1.0);
+    return __builtin_fma(rem, r, r);
+}
+
+BT_DEV double cr_sqrt(double x)
+{
+    if (sqrt_in_plain_range(x)) return sqrt_plain_range(x);
+    return __builtin_sqrt(x);
+}
+
+BT_DEV double cr_rsqrt(double x)                    // 1/sqrt(x): two roundings
+{
+    if (sqrt_in_plain_range(x)) return recip_plain_range(sqrt_plain_range(x));
+    return 1.0 / __builtin_sqrt(x);
+}
+
+// ---------------------------------------------------------------------------
 // Bicubic table (src/table.cpp:42-158)
 // ---------------------------------------------------------------------------
 BT_DEV double spline1(double x, double v0, double v1, double d0, double d1)
@@ -188,7 +251,7 @@ BT_DEV double quadric_sag(const bt_surface* s, double x, double y)
     const double R = s->p[0];
     double r2 = x*x + y*y;
     if (R != 0)
-        return r2/(R*(1.+sqrt(1.-r2*s->p[8])));
+        return r2/(R*(1.+cr_sqrt(1.-r2*s->p[8])));
     return 0.0;
 }
 
@@ -196,7 +259,7 @@ BT_DEV double quadric_dzdr(const bt_surface* s, double r)
 {
     const double R = s->p[0];
     if (R != 0.0)
-        return r/(R*sqrt(1-r*r*s->p[8]));
+        return r/(R*cr_sqrt(1-r*r*s->p[8]));
     return 0.0;
 }
 
@@ -234,12 +297,12 @@ template <bool kAsphere>
 BT_DEV void conic_normal(const bt_surface* s, double x, double y,
                          double& nx, double& ny, double& nz)
 {
-    double r = sqrt(x*x + y*y);
+    double r = cr_sqrt(x*x + y*y);
     if (r == 0.0) {
         nx = 0.0; ny = 0.0; nz = 1.0;
     } else {
         double dzdr = kAsphere ? asphere_dzdr(s, r) : quadric_dzdr(s, r);
-        nz = 1/sqrt(1+dzdr*dzdr);
+        nz = cr_rsqrt(1+dzdr*dzdr);
         nx = -x/r*dzdr*nz;
         ny = -y/r*dzdr*nz;
     }
@@ -255,7 +318,7 @@ BT_DEV void poly_normal(const bt_surface* s, double x, double y,
 {                                                    // polynomialSurface.cpp:39-48
     double gx = -horner2d(x-s->p[0], y-s->p[1], s->a1, s->n1-1, s->n2-1);
     double gy = -horner2d(x-s->p[0], y-s->p[1], s->a2, s->n1-1, s->n2-1);
-    nz = 1./sqrt(gx*gx + gy*gy + 1);
+    nz = cr_rsqrt(gx*gx + gy*gy + 1);
     nx = gx*nz;
     ny = gy*nz;
 }
@@ -269,7 +332,7 @@ BT_DEV void table_normal(const bt_surface* s, double x, double y,
         nx = __builtin_nan(""); ny = nx; nz = nx;
         return;
     }
-    nz = 1/sqrt(1 + dxdz*dxdz + dydz*dydz);
+    nz = cr_rsqrt(1 + dxdz*dxdz + dydz*dydz);
     nx = -dxdz*nz;
     ny = -dydz*nz;
 }
@@ -285,7 +348,7 @@ BT_DEV void table_sag_normal(const bt_surface* s, double x, double y, double& sa
         nx = __builtin_nan(""); ny = nx; nz = nx;
         return;
     }
-    nz = 1/sqrt(1 + dxdz*dxdz + dydz*dydz);
+    nz = cr_rsqrt(1 + dxdz*dxdz + dydz*dydz);
     nx = -dxdz*nz;
     ny = -dydz*nz;
 }
@@ -338,7 +401,7 @@ BT_DEV double leaf_sag(const bt_surface* s, double x, double y)
         return 0.0;
     case BT_SPHERE:                                  // sphere.cpp:17-21
         if (s->p[0] != 0)
-            return s->p[0]*(1-sqrt(1-(x*x + y*y)*s->p[3]));
+            return s->p[0]*(1-cr_sqrt(1-(x*x + y*y)*s->p[3]));
         return 0.0;
     case BT_PARABOLOID:                              // paraboloid.cpp:17-23
         if (s->p[0] != 0) {
@@ -375,10 +438,10 @@ BT_DEV void leaf_normal(const bt_surface* s, double x, double y,
         if (s->p[0] == 0.0 || rsqr == 0.0) {
             nx = 0.0; ny = 0.0; nz = 1.0;
         } else {
-            double r = sqrt(rsqr);
+            double r = cr_sqrt(rsqr);
             double rat = r*s->p[2];
-            double dzdr = rat/sqrt(1-rat*rat);
-            nz = 1/sqrt(1+dzdr*dzdr);
+            double dzdr = rat/cr_sqrt(1-rat*rat);
+            nz = cr_rsqrt(1+dzdr*dzdr);
             nx = -x/r*dzdr*nz;
             ny = -y/r*dzdr*nz;
         }
@@ -389,7 +452,7 @@ BT_DEV void leaf_normal(const bt_surface* s, double x, double y,
             nx = 0.0; ny = 0.0; nz = 1.0;
         } else {
             const double Rinv = s->p[1];
-            nz = 1/sqrt(1+(x*x+y*y)*Rinv*Rinv);
+            nz = cr_rsqrt(1+(x*x+y*y)*Rinv*Rinv);
             nx = -x*Rinv*nz;
             ny = -y*Rinv*nz;
         }
@@ -397,13 +460,13 @@ BT_DEV void leaf_normal(const bt_surface* s, double x, double y,
     }
     case BT_QUADRIC:                                 // quadric.cpp:27-39
     case BT_ASPHERE: {                               // asphere.cpp:48-62
-        double r = sqrt(x*x + y*y);
+        double r = cr_sqrt(x*x + y*y);
         if (r == 0.0) {
             nx = 0.0; ny = 0.0; nz = 1.0;
         } else {
             double dzdr = (s->kind == BT_QUADRIC) ? quadric_dzdr(s, r)
                                                   : asphere_dzdr(s, r);
-            nz = 1/sqrt(1+dzdr*dzdr);
+            nz = cr_rsqrt(1+dzdr*dzdr);
             nx = -x/r*dzdr*nz;
             ny = -y/r*dzdr*nz;
         }
@@ -413,7 +476,7 @@ BT_DEV void leaf_normal(const bt_surface* s, double x, double y,
         const double tx = s->p[0], ty = s->p[1];
         nx = -tx;
         ny = -ty;
-        nz = sqrt(1 - tx*tx - ty*ty);
+        nz = cr_sqrt(1 - tx*tx - ty*ty);
         return;
     }
     case BT_BICUBIC:                                 // bicubic.cpp:23-40
@@ -448,7 +511,7 @@ BT_DEV void add_slope(double& sx, double& sy, double nx, double ny, double nz)
 
 BT_DEV void sum_normal_from_slopes(double sx, double sy, double& nx, double& ny, double& nz)
 {
-    nz = 1./sqrt(sx*sx + sy*sy + 1);
+    nz = cr_rsqrt(sx*sx + sy*sy + 1);
     nx = sx*nz;
     ny = sy*nz;
 }
@@ -795,9 +858,9 @@ BT_DEV bool quadric_tti(const bt_surface* s, double x, double y, double z,
         return false;
     double dt1;
     if (b > 0)
-        dt1 = (-b - sqrt(discriminant)) / (2*a);
+        dt1 = (-b - cr_sqrt(discriminant)) / (2*a);
     else
-        dt1 = 2*c / (-b + sqrt(discriminant));
+        dt1 = 2*c / (-b + cr_sqrt(discriminant));
     double dt2 = c / (a*dt1);
     double z1 = z + vz*dt1;
     double z2 = z + vz*dt2;
@@ -829,9 +892,9 @@ BT_DEV bool leaf_tti(const bt_surface* s, double x, double y, double z,
         if (discriminant < 0) return false;
         double dt1;
         if (b > 0)
-            dt1 = (-b - sqrt(discriminant)) / (2*a);
+            dt1 = (-b - cr_sqrt(discriminant)) / (2*a);
         else
-            dt1 = 2*c / (-b + sqrt(discriminant));
+            dt1 = 2*c / (-b + cr_sqrt(discriminant));
         double dt2 = c / (a*dt1);
         dt = pick_root(vz*R, dt1, dt2);
         return true;
@@ -848,9 +911,9 @@ BT_DEV bool leaf_tti(const bt_surface* s, double x, double y, double z,
         if (discriminant < 0) return false;
         double dt1;
         if (b > 0)
-            dt1 = (-b - sqrt(discriminant)) / (2*a);
+            dt1 = (-b - cr_sqrt(discriminant)) / (2*a);
         else
-            dt1 = 2*c / (-b + sqrt(discriminant));
+            dt1 = 2*c / (-b + cr_sqrt(discriminant));
         double dt2 = c / (a*dt1);
         dt = pick_root(vz*s->p[0], dt1, dt2);
         return true;
@@ -1035,13 +1098,13 @@ BT_DEV double medium_n(const bt_medium* m, double w)
     case BT_MED_SELLMEIER: {                         // medium.cpp:145-149
         const double* p = m->p;
         double x = w*w*1e12;
-        return sqrt(1.0 + p[0]*x/(x-p[3]) + p[1]*x/(x-p[4]) + p[2]*x/(x-p[5]));
+        return cr_sqrt(1.0 + p[0]*x/(x-p[3]) + p[1]*x/(x-p[4]) + p[2]*x/(x-p[5]));
     }
     case BT_MED_SUMITA: {                            // medium.cpp:186-191
         const double* A = m->p;
         double x = w*w*1e12;
         double y = 1./x;
-        return sqrt(A[0] + A[1]*x + y*(A[2] + y*(A[3] + y*(A[4] + y*A[5]))));
+        return cr_sqrt(A[0] + A[1]*x + y*(A[2] + y*(A[3] + y*(A[4] + y*A[5]))));
     }
     case BT_MED_AIR: {                               // medium.cpp:237-245
         const double P = m->p[0], T = m->p[1], W = m->p[2];
@@ -1069,7 +1132,7 @@ BT_DEV double medium_n(const bt_medium* m, double w)
 // ---------------------------------------------------------------------------
 BT_DEV double hypot_kernel(double ax, double ay)    // ax >= ay >= 0
 {
-    double h = sqrt(ax*ax + ay*ay);
+    double h = cr_sqrt(ax*ax + ay*ay);
     double t1, t2;
     if (h <= 2.0*ay) {
         double delta = h - ay;
@@ -1293,7 +1356,7 @@ BT_DEV bool apply_interaction(int action, const double* __restrict__ dr,
         double n1 = vx*vx;
         n1 += vy*vy;
         n1 += vz*vz;
-        n1 = 1/sqrt(n1);
+        n1 = cr_rsqrt(n1);
         double nvx = vx*n1;
         double nvy = vy*n1;
         double nvz = vz*n1;
@@ -1313,7 +1376,7 @@ BT_DEV bool apply_interaction(int action, const double* __restrict__ dr,
             r.fail = true;
             r.vig = true;
         } else {
-            double nfactor = eta*alpha + sqrt(1-sinsqr);
+            double nfactor = eta*alpha + cr_sqrt(1-sinsqr);
             vx = (eta*nvx - nfactor*nx) / n2;
             vy = (eta*nvy - nfactor*ny) / n2;
             vz = (eta*nvz - nfactor*nz) / n2;

@@ -62,7 +62,7 @@ __global__ __launch_bounds__(kBlock) void k_rsplit(
             double n1 = vx*vx;
             n1 += vy*vy;
             n1 += vz*vz;
-            n1 = 1/sqrt(n1);
+            n1 = cr_rsqrt(n1);
             double nvx = vx*n1;
             double nvy = vy*n1;
             double nvz = vz*n1;
@@ -105,7 +105,7 @@ __global__ __launch_bounds__(kBlock) void k_rsplit(
                 rv.vignetted[i] = 1;
                 rvig = true;
             } else {
-                const double nfactor = eta*alpha + sqrt(1-sinsqr);
+                const double nfactor = eta*alpha + cr_sqrt(1-sinsqr);
                 double ux = eta*nvx - nfactor*nx;
                 double uy = eta*nvy - nfactor*ny;
                 double uz = eta*nvz - nfactor*nz;

@@ -361,6 +361,15 @@ __global__ __launch_bounds__(kBlock) void k_contains(
     out[i] = obsc_contains(o, len, x[i], y[i]) ? 1 : 0;
 }
 
+__global__ __launch_bounds__(kBlock) void k_check_sqrt(
+    const double* __restrict__ x, double* __restrict__ out, int64_t n)
+{
+    const int64_t i = ray_index();
+    if (i >= n) return;
+    out[i] = cr_sqrt(x[i]);
+    out[n + i] = cr_rsqrt(x[i]);
+}
+
 // ---------------------------------------------------------------------------
 // Host helpers
 // ---------------------------------------------------------------------------
@@ -675,6 +684,16 @@ int bt_obsc_contains(const bt_obsc* obsc, int obsc_len, const double* x,
     hipLaunchKernelGGL(k_contains, dim3(grid_for(n)), dim3(kBlock), 0, as_stream(stream),
                        obsc, obsc_len, x, y, out, n);
     return check_launch("bt_obsc_contains");
+}
+
+int bt_check_sqrt(const double* x, double* out, int64_t n, void* stream)
+{
+    if (n > 0 && (!x || !out)) return fail(BT_E_NULL, "bt_check_sqrt: null argument");
+    if (n < 0) return fail(BT_E_SIZE, "bt_check_sqrt: negative n");
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(k_check_sqrt, dim3(grid_for(n)), dim3(kBlock), 0, as_stream(stream),
+                       x, out, n);
+    return check_launch("bt_check_sqrt");
 }
 
 }  // extern "C"

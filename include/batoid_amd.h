@@ -405,6 +405,13 @@ int bt_medium_get_n(const bt_medium* medium, const double* wavelength,
 int bt_obsc_contains(const bt_obsc* obsc, int obsc_len, const double* x,
                      const double* y, uint8_t* out, int64_t n, void* stream);
 
+/* The square roots every kernel of the trace uses: out[0:n] = sqrt(x) and
+ * out[n:2n] = 1/sqrt(x) as the trace computes them (x, out device arrays;
+ * out holds 2n doubles).  They must equal IEEE-754 double sqrt and division
+ * bit for bit, as the reference's CPU arithmetic does; this entry point
+ * exists to test that and has no counterpart in the reference.            */
+int bt_check_sqrt(const double* x, double* out, int64_t n, void* stream);
+
 /* ---- RayVector field operations -----------------------------------------
  * Replace CPPRayVector.positionAtTime / propagateInPlace / phase /
  * amplitude / sumAmplitude (src/rayVector.cpp:26-222,
