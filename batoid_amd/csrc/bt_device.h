@@ -14,6 +14,7 @@
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <type_traits>
 #include "../../include/batoid_amd.h"
 #include "bt_mlp.h"
 
@@ -1369,18 +1370,35 @@ BT_DEV bool apply_interaction(int action, const double* __restrict__ dr,
             nx *= -1; ny *= -1; nz *= -1;
             alpha *= -1;
         }
-        double eta = n1/n2;
-        double sinsqr = eta*eta*(1-alpha*alpha);
-        if (sinsqr > 1.0) {                          // total internal reflection
-            vx = __builtin_nan(""); vy = vx; vz = vx;
-            r.fail = true;
-            r.vig = true;
-        } else {
-            double nfactor = eta*alpha + cr_sqrt(1-sinsqr);
-            vx = (eta*nvx - nfactor*nx) / n2;
-            vy = (eta*nvy - nfactor*ny) / n2;
-            vz = (eta*nvz - nfactor*nz) / n2;
-        }
+        // Into a medium of index exactly 1 (vacuum, or air given as a
+        // constant 1) the reference's divisions by n2 return their numerators
+        // bit for bit (x/1 == x in IEEE arithmetic, signed zeros, infinities
+        // and NaNs included), so that case runs without them.
+        auto finish = [&](auto unit) {
+            constexpr bool kUnit = decltype(unit)::value;
+            double eta = n1;
+            if constexpr (!kUnit) eta /= n2;
+            double sinsqr = eta*eta*(1-alpha*alpha);
+            if (sinsqr > 1.0) {                      // total internal reflection
+                vx = __builtin_nan(""); vy = vx; vz = vx;
+                r.fail = true;
+                r.vig = true;
+            } else {
+                double nfactor = eta*alpha + cr_sqrt(1-sinsqr);
+                vx = eta*nvx - nfactor*nx;
+                vy = eta*nvy - nfactor*ny;
+                vz = eta*nvz - nfactor*nz;
+                if constexpr (!kUnit) {
+                    vx /= n2;
+                    vy /= n2;
+                    vz /= n2;
+                }
+            }
+        };
+        if (n2 == 1.0)
+            finish(std::true_type{});
+        else
+            finish(std::false_type{});
     }
     r.x = x; r.y = y; r.z = z;
     r.vx = vx; r.vy = vy; r.vz = vz;
