@@ -46,6 +46,14 @@ def f64(a):
     return torch.from_numpy(arr).to(device())
 
 
+def i32(a):
+    """A contiguous int32 device copy of numpy array ``a``."""
+    arr = np.ascontiguousarray(a, dtype=np.int32)
+    if not arr.flags.writeable:
+        arr = arr.copy()
+    return torch.from_numpy(arr).to(device())
+
+
 def u8(a):
     if isinstance(a, torch.Tensor):
         return a.to(device=device(), dtype=torch.uint8).contiguous()
@@ -88,7 +96,7 @@ class Materialized:
             for k, a in enumerate(arrs):
                 if a is None:
                     continue
-                t = f64(a)
+                t = i32(a) if getattr(a, "dtype", None) == np.int32 else f64(a)
                 self.tables.append(t)
                 setattr(rec, f"a{k}", t.data_ptr())
             recs.append(rec)

@@ -199,6 +199,40 @@ BT_DEV double horner2d(double x, double y, const double* __restrict__ c,
     return r;
 }
 
+// horner2d over trimmed bounds.  `trim` holds the last row with a
+// coefficient other than +0.0 (-1 if none), then each row's length up to
+// its last such coefficient.  The full loop starts each row (and the outer
+// sum) from 0.0 and, over a run of +0.0 coefficients with a finite argument,
+// stays exactly +0.0: (+-0)*y + (+0) == +0.  Starting after such a run from
+// 0.0 therefore gives the same bits, and rows of only +0.0 add exactly +0.0
+// where the full loop does.  A wave with any non-finite argument (0*inf is
+// NaN inside the skipped steps) runs the full loop.
+BT_DEV double horner2d_trim(double x, double y, const double* __restrict__ c,
+                            int nx, int ny, const int* __restrict__ trim)
+{
+    if (!trim || !__all(__builtin_isfinite(x) && __builtin_isfinite(y)))
+        return horner2d(x, y, c, nx, ny);
+    double r = 0.0;
+    for (int i = trim[0]; i >= 0; i--) {
+        const double* row = c + (int64_t)i*nx;
+        double h = 0.0;
+        for (int j = trim[1 + i] - 1; j >= 0; j--) { h *= y; h += row[j]; }
+        r *= x;
+        r += h;
+    }
+    return r;
+}
+
+// Trim tables of a POLYNOMIAL record's coefficient, x-gradient and
+// y-gradient tables, in that order, in the record's optional int32 table a3.
+BT_DEV const int* poly_trim(const bt_surface* s, int which)
+{
+    const int* t = reinterpret_cast<const int*>(s->a3);
+    if (!t) return nullptr;
+    if (which == 0) return t;
+    return t + (1 + s->n2) + (which - 1) * s->n2;
+}
+
 // table_eval and table_grad at one point, sharing the cell and its 16 table
 // values; each result is bit-identical to the separate function's.
 BT_DEV void table_eval_grad(const bt_surface* s, double x, double y, double& val,
@@ -311,14 +345,14 @@ BT_DEV void conic_normal(const bt_surface* s, double x, double y,
 
 BT_DEV double poly_sag(const bt_surface* s, double x, double y)
 {                                                    // polynomialSurface.cpp:35-37
-    return horner2d(x-s->p[0], y-s->p[1], s->a0, s->n1, s->n2);
+    return horner2d_trim(x-s->p[0], y-s->p[1], s->a0, s->n1, s->n2, poly_trim(s, 0));
 }
 
 BT_DEV void poly_normal(const bt_surface* s, double x, double y,
                         double& nx, double& ny, double& nz)
 {                                                    // polynomialSurface.cpp:39-48
-    double gx = -horner2d(x-s->p[0], y-s->p[1], s->a1, s->n1-1, s->n2-1);
-    double gy = -horner2d(x-s->p[0], y-s->p[1], s->a2, s->n1-1, s->n2-1);
+    double gx = -horner2d_trim(x-s->p[0], y-s->p[1], s->a1, s->n1-1, s->n2-1, poly_trim(s, 1));
+    double gy = -horner2d_trim(x-s->p[0], y-s->p[1], s->a2, s->n1-1, s->n2-1, poly_trim(s, 2));
     nz = cr_rsqrt(gx*gx + gy*gy + 1);
     nx = gx*nz;
     ny = gy*nz;

@@ -245,6 +245,26 @@ class Asphere(Surface):
         return out + (")" if self.imin == 2 else f", imin={self.imin})")
 
 
+def _horner_trim(tables):
+    """Trim table of 2-D Horner coefficient tables, as the device reads each
+    one (row i = flat[i*nx:(i+1)*nx], ny rows): its last row holding a
+    coefficient other than +0.0 (-1 if none), then every row's length up to
+    its last such coefficient; the tables' entries concatenated (int32).
+    Only +0.0 counts as trimmable: -0.0 is kept, since the loop's result can
+    depend on it."""
+    out = []
+    for flat, nx, ny in tables:
+        flat = np.ascontiguousarray(flat, dtype=np.float64).ravel()
+        if ny > 0 and nx > 0:
+            rows = flat[:nx * ny].reshape(ny, nx).view(np.uint64) != 0
+            lens = [int(np.flatnonzero(row)[-1]) + 1 if row.any() else 0 for row in rows]
+        else:
+            lens = [0] * max(ny, 0)
+        last = max([i for i, n in enumerate(lens) if n > 0], default=-1)
+        out += [last] + lens
+    return np.array(out, dtype=np.int32)
+
+
 class Zernike(Surface):
     """Annular Zernike series sum_j coef[j] Z_j(x/R_outer, y/R_outer) in
     Noll order (surface.py:433-518), evaluated as a 2-D polynomial
@@ -263,9 +283,11 @@ class Zernike(Surface):
 
     def _leaf_record(self):
         c = self._xycoef
-        return (_rec(_lib.POLYNOMIAL, [self.x_origin, self.y_origin],
-                     n1=c.shape[0], n2=c.shape[1]),
-                (c, self._xycoef_gradx, self._xycoef_grady, None))
+        n1, n2 = c.shape
+        trim = _horner_trim([(c, n1, n2), (self._xycoef_gradx, n1 - 1, n2 - 1),
+                             (self._xycoef_grady, n1 - 1, n2 - 1)])
+        return (_rec(_lib.POLYNOMIAL, [self.x_origin, self.y_origin], n1=n1, n2=n2),
+                (c, self._xycoef_gradx, self._xycoef_grady, trim))
 
     def __hash__(self):
         return hash(("batoid.Zernike", tuple(self.coef), self.R_outer, self.R_inner,

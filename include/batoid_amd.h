@@ -75,6 +75,11 @@ typedef struct bt_rays {
  *   POLYNOMIAL   p = {xorigin, yorigin}; n1 = xsize, n2 = ysize;
  *                a0 = coefs[ysize][xsize], a1 = gradx, a2 = grady
  *                ([ysize-1][xsize-1])                     polynomialSurface.cpp:35-66
+ *                a3 = optional int32 trim table (or NULL): for a0, a1, a2
+ *                in turn, the last row holding a coefficient other than
+ *                +0.0 (-1 if none), then each row's length up to its last
+ *                such coefficient (1 + n2, n2, n2 entries).  Evaluation
+ *                skips the +0.0 runs it marks, with bit-identical results.
  *   SUM          nchild children follow; n1 = records below   sum.cpp:23-56
  */
 enum bt_surface_kind {

@@ -150,7 +150,8 @@ class _HostTable:
             for k, a in enumerate(arrs):
                 if a is None:
                     continue
-                a = np.ascontiguousarray(a, dtype=float)
+                a = np.ascontiguousarray(a, dtype=np.int32 if getattr(a, "dtype", None) == np.int32
+                                         else float)
                 self.keep.append(a)
                 setattr(rec, f"a{k}", a.ctypes.data)
             recs.append(rec)

@@ -14,7 +14,7 @@ from oracle.host import HostRays, HostPlan, host_records, nthreads
 from batoid_amd import _lib
 from batoid_amd.plan import PlanSpec, collect_steps
 from helpers import (surfaces, media, obscurations, random_rays, transforms,
-                     assert_cols_identical, FCOLS, BCOLS)
+                     assert_cols_identical, zernike_surface, bicubic_surface, FCOLS, BCOLS)
 
 pytestmark = pytest.mark.gpu
 
@@ -349,3 +349,76 @@ def test_sum_layout_flag_on_other_sums_fails_rays():
     st.flags |= _lib.PLAN_SUM_APB
     _lib.call("bt_trace_plan", st, rv._rays, rv._rays, NITER, _device.stream())
     assert rv.failed.all() and rv.vignetted.all()
+
+
+def _crafted_zernikes():
+    """Zernike records whose coefficient tables exercise the trimmed Horner
+    loops: the triangular tables of a real series, tables with -0.0 entries
+    (kept), rows of only +0.0 inside and at the top, a row of only -0.0, and
+    an all-zero table."""
+    rng = np.random.default_rng(77)
+    out = []
+    base = zernike_surface(seed=71, jmax=28)
+    out.append(("triangular", base))
+    z = zernike_surface(seed=72, jmax=28)
+    c, gx, gy = (t.copy() for t in (z._xycoef, z._xycoef_gradx, z._xycoef_grady))
+    c[2, :] = 0.0                      # a row of +0.0 inside
+    c[6, :] = 0.0                      # the top row: +0.0 only
+    c[5, :] = 0.0
+    c[4, 3] = -0.0                     # a -0.0 beyond the row's last nonzero
+    c[1, 6] = -0.0
+    gx[3, :] = -0.0                    # a row of only -0.0
+    gy[5, :] = 0.0
+    gy[4, :] = 0.0
+    gy[0, 5] = rng.normal() * 1e-4     # a nonzero past the triangle
+    z._xycoef, z._xycoef_gradx, z._xycoef_grady = c, gx, gy
+    out.append(("signed_zeros", z))
+    z = zernike_surface(seed=73, jmax=28)
+    z._xycoef = np.zeros_like(z._xycoef)
+    z._xycoef_gradx = np.zeros_like(z._xycoef_gradx)
+    z._xycoef_grady = np.zeros_like(z._xycoef_grady)
+    out.append(("all_zero", z))
+    return out
+
+
+def test_polynomial_trim_bit_exact():
+    """Polynomial sags and normals skip the +0.0 runs of their coefficient
+    tables; the oracle runs the reference's full loops.  Points include
+    +-0, NaN, +-inf and huge coordinates, alone in their waves and mixed
+    into waves of ordinary points (any non-finite lane sends its wave
+    through the full loop)."""
+    O = oracle.load()
+    rng = np.random.default_rng(78)
+    x, y = rng.uniform(-1.5, 1.5, N), rng.uniform(-1.5, 1.5, N)
+    special = np.array([0.0, -0.0, np.nan, np.inf, -np.inf, 1e300, -1e-300, 5e-324])
+    k = len(special)
+    x[:k * k] = np.repeat(special, k)          # every pair, in the first waves
+    y[:k * k] = np.tile(special, k)
+    idx = rng.choice(np.arange(256, N), 64, replace=False)
+    x[idx] = special[rng.integers(0, k, 64)]   # stray ones in ordinary waves
+    y[idx[::2]] = special[rng.integers(0, k, 32)]
+    for name, surf in _crafted_zernikes():
+        trim = surf._leaf_record()[1][3]
+        assert trim is not None and trim.dtype == np.int32
+        tab = host_records(surf)
+        s = np.empty(N)
+        nrm = np.empty(3 * N)
+        with np.errstate(all="ignore"):
+            O.or_sag_n(tab.ptr(), x.ctypes.data, y.ctypes.data, s.ctypes.data, N, nthreads())
+            O.or_normal_n(tab.ptr(), x.ctypes.data, y.ctypes.data, nrm.ctypes.data, N,
+                          nthreads())
+        assert_cols_identical({"s": surf.sag(x, y)}, {"s": s}, cols=("s",), what=name)
+        g = surf.normal(x, y)
+        assert_cols_identical({"n": g.T.ravel()}, {"n": nrm}, cols=("n",), what=name)
+        # the same surfaces inside a Sum, traced by the plan kernel
+        optic = bt.CompoundOptic([
+            bt.Mirror(bt.Sum(bt.Asphere(9.0, -0.9, [1e-4]), surf, bicubic_surface(seed=9)),
+                      name="M", coordSys=bt.CoordSys()),
+            bt.Detector(bt.Plane(), name="D", coordSys=bt.CoordSys().shiftLocal([0, 0, 2.5]))],
+            name=name, inMedium=bt.vacuum)
+        base = HostRays(**random_rays(N, seed=79, z0=2.0, spread=0.05, radius=1.1))
+        ref = HostPlan(PlanSpec(collect_steps(optic), bt.CoordSys())).trace(base.copy(),
+                                                                             niter=NITER)
+        rv = _gpu_rays(base)
+        optic.trace(rv)
+        assert_cols_identical(_host(rv), ref.cols, what=f"plan {name}")
