@@ -34,15 +34,15 @@ BYTES_PER_RAY = 8 * 8 + 2 + 7 * 8 + 2
 # rocprofv3 FETCH_SIZE and WRITE_SIZE (each scaled by its reading for a
 # 1 GiB device copy: x2.000 and x1.000), from
 # profiles/trace_plan_tma_1e7_counters.md.
-MEASURED_TRAFFIC_PER_RAY = 124.05
+MEASURED_TRAFFIC_PER_RAY = 124.06
 # FP64 flops per ray of the same kernel on this workload: 64 lanes x
 # (ADD + MUL + 2 FMA) FP64 wave instructions (SQ_INSTS_VALU_*_F64), same file
 # (current build's figure; issued lanes, an upper bound on useful flops).
-FP64_FLOP_PER_RAY = 6457.6
+FP64_FLOP_PER_RAY = 6220.1
 # The same two figures for the C3 kernel (perturbed TMA, reference start),
 # from profiles/trace_plan_c3_perturbed_tma_1e7.md.
-C3_FP64_FLOP_PER_RAY = 19839.0
-C3_TRAFFIC_PER_RAY = 124.87
+C3_FP64_FLOP_PER_RAY = 18232.3
+C3_TRAFFIC_PER_RAY = 124.85
 # Ray generation writes 9 FP64 columns and 2 flag bytes per ray.
 RAYGEN_BYTES_PER_RAY = 9 * 8 + 2
 
