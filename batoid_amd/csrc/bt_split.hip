@@ -23,10 +23,16 @@ namespace bt {
 __global__ __launch_bounds__(kBlock) void k_rsplit(
     Xform xf, const bt_surface* __restrict__ surf, const bt_medium* __restrict__ m2,
     double reflect, double transmit, const bt_obsc* __restrict__ obsc, int olen, bt_rays rv,
-    bt_rays sp, int niter)
+    bt_rays sp, int niter, int live, double min_flux)
 {                                                    // batoid.cpp:582-672
     const int64_t i = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (i >= rv.n) return;
+    // traceSplit's filter (optic.py:1262) has already dropped this ray: it
+    // stays as it is, and its split part is only marked dropped too
+    if (live && (rv.vignetted[i] != 0 || !(rv.flux[i] >= min_flux))) {
+        sp.vignetted[i] = 1;
+        return;
+    }
     const double* dr = xf.dr;
     const double* drot = xf.drot;
     const double x0 = rv.x[i], y0 = rv.y[i], z0 = rv.z[i];
@@ -244,24 +250,46 @@ using namespace bt;
 
 extern "C" {
 
+namespace {
+int rsplit_launch(const bt_surface* surface, const double* dr, const double* drot,
+                  const bt_medium* m2, const bt_coating* coating, const bt_obsc* obsc,
+                  int obsc_len, bt_rays rays, bt_rays split, int niter, int live,
+                  double min_flux, void* stream, const char* name)
+{
+    const std::string n(name);
+    if (!surface || !dr || !drot || !m2 || !coating)
+        return fail(BT_E_NULL, n + ": null argument");
+    if (obsc_len < 0 || (obsc_len > 0 && !obsc))
+        return fail(BT_E_SIZE, n + ": bad obscuration program");
+    if (niter < 0) return fail(BT_E_SIZE, n + ": negative niter");
+    if (rays.n < 0 || split.n < rays.n)
+        return fail(BT_E_SIZE, n + ": split holds fewer rays");
+    if (rays.n == 0) return 0;
+    if (!rays_ok(rays) || !rays_ok(split))
+        return fail(BT_E_NULL, n + ": incomplete ray columns");
+    hipLaunchKernelGGL(k_rsplit, dim3(grid_for(rays.n)), dim3(kBlock), 0, as_stream(stream),
+                       make_xform(dr, drot), surface, m2, coating->reflectivity,
+                       coating->transmissivity, obsc, obsc_len, rays, split, niter, live,
+                       min_flux);
+    return check_launch(name);
+}
+}  // namespace
+
 int bt_rsplit(const bt_surface* surface, const double* dr, const double* drot,
               const bt_medium* m2, const bt_coating* coating, const bt_obsc* obsc,
               int obsc_len, bt_rays rays, bt_rays split, int niter, void* stream)
 {
-    if (!surface || !dr || !drot || !m2 || !coating)
-        return fail(BT_E_NULL, "bt_rsplit: null argument");
-    if (obsc_len < 0 || (obsc_len > 0 && !obsc))
-        return fail(BT_E_SIZE, "bt_rsplit: bad obscuration program");
-    if (niter < 0) return fail(BT_E_SIZE, "bt_rsplit: negative niter");
-    if (rays.n < 0 || split.n < rays.n)
-        return fail(BT_E_SIZE, "bt_rsplit: split holds fewer rays");
-    if (rays.n == 0) return 0;
-    if (!rays_ok(rays) || !rays_ok(split))
-        return fail(BT_E_NULL, "bt_rsplit: incomplete ray columns");
-    hipLaunchKernelGGL(k_rsplit, dim3(grid_for(rays.n)), dim3(kBlock), 0, as_stream(stream),
-                       make_xform(dr, drot), surface, m2, coating->reflectivity,
-                       coating->transmissivity, obsc, obsc_len, rays, split, niter);
-    return check_launch("bt_rsplit");
+    return rsplit_launch(surface, dr, drot, m2, coating, obsc, obsc_len, rays, split, niter, 0,
+                         0.0, stream, "bt_rsplit");
+}
+
+int bt_rsplit_live(const bt_surface* surface, const double* dr, const double* drot,
+                   const bt_medium* m2, const bt_coating* coating, const bt_obsc* obsc,
+                   int obsc_len, bt_rays rays, bt_rays split, double min_flux, int niter,
+                   void* stream)
+{
+    return rsplit_launch(surface, dr, drot, m2, coating, obsc, obsc_len, rays, split, niter, 1,
+                         min_flux, stream, "bt_rsplit_live");
 }
 
 int64_t bt_select_workspace_bytes(int64_t n)

@@ -150,19 +150,25 @@ class Interface(Optic):
         if self.name not in unless:
             self.obscuration = None
 
-    def traceSplit(self, rv, minFlux=1e-3, reverse=False, _verbose=False):
+    def traceSplit(self, rv, minFlux=1e-3, reverse=False, _verbose=False, _filtered=False):
         """Split rays at this Interface into those continuing forward and
         those sent back (optic.py:466-535); returns two lists of
         RayVectors (None entries where a direction gets no rays), each
-        tagged with its ``path``."""
+        tagged with its ``path``.
+
+        ``_filtered`` (CompoundOptic.traceSplit's own use): ``rv`` still
+        holds the rays traceSplit's filter dropped after the previous item
+        (vignetted, or flux < minFlux); they are carried along, untouched,
+        instead of being copied out."""
         if _verbose:
             s = "forward" if not reverse else "reverse"
             print("traceSplit {}       {:30s} flux = {:18.8f}   nphot = {:10d}".format(
-                s, self.name, float(rv.torch("flux").sum()), len(rv)))
+                s, self.name, *_live_stats(rv, minFlux if _filtered else None)))
         if self.skip:
             return (None, rv) if reverse else (rv, None)
         # the obscuration applies to both directions, in the same pass
-        refracted, reflected = self.rSplit(rv, reverse=reverse, obscuration=self.obscuration)
+        refracted, reflected = self.rSplit(rv, reverse=reverse, obscuration=self.obscuration,
+                                           _live_min_flux=minFlux if _filtered else None)
         path = list(getattr(rv, 'path', [])) + [self.name]
         for r in (refracted, reflected):
             if r is not None:
@@ -175,9 +181,9 @@ class Interface(Optic):
         return rForward, rReverse
 
     def _split_batch(self, inputs, minFlux, _verbose=False):
-        """traceSplit of several (rays, reverse) inputs (no host
+        """traceSplit of several (rays, reverse, filtered) inputs (no host
         synchronisation: the launches queue on the stream)."""
-        return [self.traceSplit(rv, minFlux, rev, _verbose) for rv, rev in inputs]
+        return [self.traceSplit(rv, minFlux, rev, _verbose, filt) for rv, rev, filt in inputs]
 
     @cached_property
     def R_outer(self):
@@ -244,7 +250,7 @@ class RefractiveInterface(Interface):
         self.forwardCoating = SimpleCoating(reflectivity=0.0, transmissivity=1.0)
         self.reverseCoating = SimpleCoating(reflectivity=0.0, transmissivity=1.0)
 
-    def rSplit(self, rv, reverse=False, obscuration=None):
+    def rSplit(self, rv, reverse=False, obscuration=None, _live_min_flux=None):
         """(refracted, reflected); as optic.py:786-797, the forward direction
         uses reverseCoating and the reverse direction forwardCoating."""
         if reverse:
@@ -252,7 +258,7 @@ class RefractiveInterface(Interface):
         else:
             m1, m2, coating = self.inMedium, self.outMedium, self.reverseCoating
         return self.surface.rSplit(rv, m1, m2, coating, coordSys=self.coordSys,
-                                   obscuration=obscuration)
+                                   obscuration=obscuration, _live_min_flux=_live_min_flux)
 
     def _interact(self, rv, reverse, obscuration):
         from .trace import refract
@@ -271,7 +277,7 @@ class Mirror(Interface):
         self.forwardCoating = SimpleCoating(reflectivity=1.0, transmissivity=0.0)
         self.reverseCoating = SimpleCoating(reflectivity=1.0, transmissivity=0.0)
 
-    def rSplit(self, rv, reverse=False, obscuration=None):
+    def rSplit(self, rv, reverse=False, obscuration=None, _live_min_flux=None):
         from .trace import reflect
         return None, reflect(self.surface, rv, self.coordSys, obscuration=obscuration)
 
@@ -291,10 +297,11 @@ class Detector(Interface):
         self.forwardCoating = SimpleCoating(reflectivity=0.0, transmissivity=1.0)
         self.reverseCoating = None
 
-    def rSplit(self, rv, reverse=False, obscuration=None):
+    def rSplit(self, rv, reverse=False, obscuration=None, _live_min_flux=None):
         assert not reverse
         return self.surface.rSplit(rv, self.inMedium, self.outMedium, self.forwardCoating,
-                                   coordSys=self.coordSys, obscuration=obscuration)
+                                   coordSys=self.coordSys, obscuration=obscuration,
+                                   _live_min_flux=_live_min_flux)
 
 
 class Baffle(Interface):
@@ -305,7 +312,7 @@ class Baffle(Interface):
         self.forwardCoating = SimpleCoating(reflectivity=0.0, transmissivity=1.0)
         self.reverseCoating = SimpleCoating(reflectivity=0.0, transmissivity=1.0)
 
-    def rSplit(self, rv, reverse=False, obscuration=None):
+    def rSplit(self, rv, reverse=False, obscuration=None, _live_min_flux=None):
         from .trace import intersect
         return intersect(self.surface, rv, self.coordSys, obscuration=obscuration), None
 
@@ -321,8 +328,18 @@ class OPDScreen(Interface):
     def _interact(self, rv, reverse, obscuration):
         raise NotImplementedError("OPDScreen is outside the GPU trace path")
 
-    def rSplit(self, rv, reverse=False, obscuration=None):
+    def rSplit(self, rv, reverse=False, obscuration=None, _live_min_flux=None):
         raise NotImplementedError("OPDScreen is outside the GPU trace path")
+
+
+def _live_stats(rv, minFlux):
+    """(flux sum, ray count) of the rays of ``rv`` traceSplit still carries:
+    all of them, or with ``minFlux`` those unvignetted with flux >= minFlux."""
+    flux = rv.torch("flux")
+    if minFlux is None:
+        return float(flux.sum()), len(rv)
+    keep = (rv.torch("vignetted") == 0) & (flux >= minFlux)
+    return float(flux[keep].sum()), int(keep.sum())
 
 
 class CompoundOptic(Optic):
@@ -497,8 +514,8 @@ class CompoundOptic(Optic):
         if _verbose:
             s = "reverse" if reverse else "forward"
             print("traceSplit {}       {:30s} flux = {:18.8f}   nphot = {:10d}".format(
-                s, self.name, float(rv.torch("flux").sum()), len(rv)))
-        return self._split_batch([(rv, reverse)], minFlux, _verbose)[0]
+                s, self.name, *_live_stats(rv, None)))
+        return self._split_batch([(rv, reverse, False)], minFlux, _verbose)[0]
 
     def _split_batch(self, inputs, minFlux, _verbose=False):
         """traceSplit of several (rays, reverse) inputs at once.
@@ -512,45 +529,55 @@ class CompoundOptic(Optic):
         parent's key plus -p for its p-th pushed child (the last pushed is
         popped first) -- and the outputs, appended while their parent is
         processed, are sorted by (parent key, position) at the end, which
-        reproduces the reference's lists exactly."""
+        reproduces the reference's lists exactly.
+
+        The filter itself (unvignetted, flux >= minFlux) is only counted
+        for the ray sets that go on to another item: they keep the dropped
+        rays, which the next items skip (``_filtered``); only the sets that
+        leave the optic are copied out, kept rays in order.  Each returned
+        set therefore holds exactly the rays the reference's per-item
+        copies would."""
         if self.skip:
-            return [((None, rv) if rev else (rv, None)) for rv, rev in inputs]
+            return [((None, rv) if rev else (rv, None)) for rv, rev, _ in inputs]
         last = len(self.items) - 1
-        # work entries: (input, key, rays, reverse, item index)
-        work = [(k, (), rv, rev, last if rev else 0) for k, (rv, rev) in enumerate(inputs)]
+        # work entries: (input, key, rays, reverse, item index, filtered)
+        work = [(k, (), rv, rev, last if rev else 0, filt)
+                for k, (rv, rev, filt) in enumerate(inputs)]
         outs = [([], []) for _ in inputs]              # per input: [(order, rays)] x 2
         while work:
             groups = {}
             for w in work:
                 groups.setdefault(w[4], []).append(w)
             work = []
-            children = []                              # (entry, side, rays)
+            children = []                              # (entry, side, rays, leaves)
             for index in sorted(groups):
                 entries = groups[index]
                 results = self.items[index]._split_batch(
-                    [(w[2], w[3]) for w in entries], minFlux, _verbose)
+                    [(w[2], w[3], w[5]) for w in entries], minFlux, _verbose)
                 for w, (rF, rR) in zip(entries, results):
                     for side, rl in enumerate((rF, rR)):
                         if not isinstance(rl, list):   # a skipped item passes rays on
                             rl = [rl]
-                        children += [(w, side, rr) for rr in rl if rr]
-            sels = RayVector._select_many([c[2] for c in children], minFlux)
+                        leaves = (side == 0 and w[4] == last) or (side == 1 and w[4] == 0)
+                        children += [(w, side, rr, leaves) for rr in rl if rr]
+            sels = RayVector._select_many([c[2] for c in children], minFlux,
+                                          compact=[c[3] for c in children])
             pushed, seq = {}, {}
-            for (w, side, rr), sel in zip(children, sels):
-                sel.path = rr.path
-                if len(sel) == 0:
+            for (w, side, rr, leaves), (sel, m) in zip(children, sels):
+                if m == 0:
                     continue
-                k, key, _, _, index = w
+                k, key, _, _, index, _ = w
                 parent = id(w)
-                if (side == 0 and index == last) or (side == 1 and index == 0):
+                if leaves:
+                    sel.path = rr.path
                     s = seq.get(parent, 0)
                     seq[parent] = s + 1
                     outs[k][side].append(((key, s), sel))
                 else:
                     p = pushed.get(parent, 0)
                     pushed[parent] = p + 1
-                    work.append((k, key + (-p,), sel, side == 1,
-                                 index + 1 if side == 0 else index - 1))
+                    work.append((k, key + (-p,), rr, side == 1,
+                                 index + 1 if side == 0 else index - 1, True))
         return [tuple([r for _, r in sorted(lst, key=lambda e: e[0])] for lst in o)
                 for o in outs]
 

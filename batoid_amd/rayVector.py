@@ -136,11 +136,15 @@ class RayVector:
         return out
 
     @staticmethod
-    def _select_many(rvs, minFlux):
-        """``[rv._select(minFlux) for rv in rvs]`` with one host
-        synchronisation for all the counts instead of one per RayVector."""
+    def _select_many(rvs, minFlux, compact=None):
+        """``[(rv._select(minFlux), count) for rv in rvs]`` with one host
+        synchronisation for all the counts instead of one per RayVector.
+        Where ``compact[i]`` is false only the count is taken: that entry
+        is ``(None, count)``."""
         if not rvs:
             return []
+        if compact is None:
+            compact = [True] * len(rvs)
         lib = _lib.load()
         stream = _device.stream()
         sizes = [int(lib.bt_select_workspace_bytes(len(rv))) for rv in rvs]
@@ -156,11 +160,14 @@ class RayVector:
         ms = counts.tolist()
         outs = []
         for i, rv in enumerate(rvs):
+            if not compact[i]:
+                outs.append((None, ms[i]))
+                continue
             out = RayVector._empty(ms[i], rv.coordSys)
             if ms[i]:
                 _lib.call("bt_select_scatter", rv._rays, float(minFlux), out._rays,
                           _device.ptr(ws[offs[i]:]), sizes[i], stream)
-            outs.append(out)
+            outs.append((out, ms[i]))
         return outs
 
     @classmethod

@@ -92,11 +92,13 @@ class Surface(DeviceCached):
         from .trace import reflect
         return reflect(self, rv, coordSys, coating)
 
-    def rSplit(self, rv, inMedium, outMedium, coating, coordSys=None, obscuration=None):
+    def rSplit(self, rv, inMedium, outMedium, coating, coordSys=None, obscuration=None,
+               _live_min_flux=None):
         """Refracted (``rv``, in place) and reflected (new) rays at this
         surface, fluxes split by ``coating`` (surface.py:107-133)."""
         from .trace import rSplit
-        return rSplit(self, rv, inMedium, outMedium, coating, coordSys, obscuration)
+        return rSplit(self, rv, inMedium, outMedium, coating, coordSys, obscuration,
+                      _live_min_flux)
 
     def refract(self, rv, inMedium, outMedium, coordSys=None, coating=None):
         """Intersect and refract rays in place (trace.py:101-116)."""

@@ -111,20 +111,28 @@ def refract(surface, rv, m1, m2, coordSys=None, coating=None, obscuration=None):
     return rv
 
 
-def rSplit(surface, rv, inMedium, outMedium, coating, coordSys=None, obscuration=None):
+def rSplit(surface, rv, inMedium, outMedium, coating, coordSys=None, obscuration=None,
+           _live_min_flux=None):
     """Split rays at ``surface`` into refracted (``rv``, in place) and
     reflected (a new RayVector) parts with the coating's flux fractions
     (trace.py:119-136, batoid.cpp:509-674).  ``obscuration``, if given, is
-    applied to both parts in the same pass.  Returns (refracted, reflected)."""
+    applied to both parts in the same pass.  Returns (refracted, reflected).
+
+    ``_live_min_flux`` (traceSplit's own use): ``rv`` still holds rays that
+    traceSplit's filter dropped (vignetted, or flux below this threshold);
+    they are skipped and only marked dropped in the reflected part."""
     from .rayVector import RayVector
     if coordSys is None:
         coordSys = rv.coordSys
     ct = CoordTransform(rv.coordSys, coordSys)
     split = RayVector._empty(len(rv), coordSys)
     optr, olen = _obsc_args(obscuration)
-    _lib.call("bt_rsplit", surface._dev.ptr(), _lib.dvec(ct.dr), _lib.dvec(ct.drot.ravel()),
-              outMedium._dev.ptr(), _coating_ptr(coating), optr, olen, rv._rays, split._rays,
-              _niter(), _device.stream())
+    args = (surface._dev.ptr(), _lib.dvec(ct.dr), _lib.dvec(ct.drot.ravel()),
+            outMedium._dev.ptr(), _coating_ptr(coating), optr, olen, rv._rays, split._rays)
+    if _live_min_flux is None:
+        _lib.call("bt_rsplit", *args, _niter(), _device.stream())
+    else:
+        _lib.call("bt_rsplit_live", *args, float(_live_min_flux), _niter(), _device.stream())
     rv.coordSys = coordSys
     return rv, split
 

@@ -370,6 +370,18 @@ int bt_refract(const bt_surface* surface, const double* dr,
 int bt_rsplit(const bt_surface* surface, const double* dr, const double* drot,
               const bt_medium* m2, const bt_coating* coating, const bt_obsc* obsc,
               int obsc_len, bt_rays rays, bt_rays split, int niter, void* stream);
+/* bt_rsplit for ray sets that traceSplit's filter (optic.py:1256-1266) has
+ * already been applied to without removing the rays it drops: rays with
+ * vignetted != 0 or flux < min_flux (NaN included) are left unchanged and
+ * only their split part is marked vignetted (its other columns are not
+ * written); every other ray is split exactly as by bt_rsplit.  Filtering
+ * the two outputs afterwards gives the rays bt_rsplit on the filtered set
+ * and the same filter would give.  The reference has no such call: its
+ * traceSplit copies out the kept rays after every Interface.              */
+int bt_rsplit_live(const bt_surface* surface, const double* dr, const double* drot,
+                   const bt_medium* m2, const bt_coating* coating, const bt_obsc* obsc,
+                   int obsc_len, bt_rays rays, bt_rays split, double min_flux, int niter,
+                   void* stream);
 
 /* ---- ray selection ------------------------------------------------------
  * The ray filter of CompoundOptic.traceSplit (optic.py:1256-1266): keep the

@@ -102,6 +102,52 @@ def test_rsplit_with_obscuration_bit_exact(name, surf):
         assert_cols_identical(_host(refl), sa.cols, what=f"{name}/rsplit+obsc reflected/{tname}")
 
 
+@pytest.mark.parametrize("name,surf", [surfaces()[i] for i in (2, 9, 15)],
+                         ids=[surfaces()[i][0] for i in (2, 9, 15)])
+def test_rsplit_live_skips_dropped_rays(name, surf):
+    """bt_rsplit_live on a set that still holds rays traceSplit's filter
+    dropped (vignetted, flux below minFlux, NaN flux): those rays are left
+    exactly as they were and their split part is marked vignetted; every
+    kept ray is split bit for bit as bt_rsplit splits it."""
+    med = media()[2][1]
+    coat = bt.SimpleCoating(0.0371, 0.9588)
+    obsc = bt.ObscNegation(bt.ObscAnnulus(0.2, 0.8))
+    rng = np.random.default_rng(29)
+    minFlux = 1e-3
+    for tname, (dr, drot) in transforms():
+        cs = _frame(dr, drot)
+        base = HostRays(**random_rays(N, seed=27, upward=0.3, spread=0.3))
+        flux = rng.uniform(0.0, 3e-3, N)
+        flux[rng.uniform(size=N) < 0.02] = np.nan
+        base.cols["flux"] = flux
+        base.cols["vignetted"] = (rng.uniform(size=N) < 0.2).astype(np.uint8)
+        keep = (base["vignetted"] == 0) & (base["flux"] >= minFlux)
+        assert 0 < keep.sum() < N
+        full_r, full_s = surf.rSplit(_gpu(base), bt.ConstMedium(1.0), med, coat, coordSys=cs,
+                                     obscuration=obsc)
+        live_r, live_s = surf.rSplit(_gpu(base), bt.ConstMedium(1.0), med, coat, coordSys=cs,
+                                     obscuration=obsc, _live_min_flux=minFlux)
+        fr, fs, lr, ls = (_host(r) for r in (full_r, full_s, live_r, live_s))
+        for k in FCOLS + BCOLS:
+            assert_cols_identical({k: lr[k][keep]}, {k: fr[k][keep]}, cols=(k,),
+                                  what=f"{name}/{tname} kept refracted")
+            assert_cols_identical({k: ls[k][keep]}, {k: fs[k][keep]}, cols=(k,),
+                                  what=f"{name}/{tname} kept reflected")
+            assert_cols_identical({k: lr[k][~keep]}, {k: base[k][~keep]}, cols=(k,),
+                                  what=f"{name}/{tname} dropped rays untouched")
+        assert ls["vignetted"][~keep].all()
+        # the filter after the split keeps the same rays either way
+        for a, b in ((live_r, full_r), (live_s, full_s)):
+            sa, sb = _host(a._select(minFlux)), _host(b._select(minFlux))
+            cnt = len(sa["x"])
+            assert cnt <= len(sb["x"])
+            kb = (fr if b is full_r else fs)
+            kept_b = (kb["vignetted"] == 0) & (kb["flux"] >= minFlux) & keep
+            assert cnt == int(kept_b.sum())
+            assert_cols_identical(sa, {k: kb[k][kept_b] for k in FCOLS + BCOLS},
+                                  what=f"{name}/{tname} filtered outputs")
+
+
 @pytest.mark.parametrize("n", [0, 1, 255, 256, 257, 100_003, 2_000_000])
 def test_select_matches_numpy(n):
     rng = np.random.default_rng(n)
