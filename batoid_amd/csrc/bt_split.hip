@@ -20,13 +20,17 @@
 
 namespace bt {
 
-__global__ __launch_bounds__(kBlock) void k_rsplit(
-    Xform xf, const bt_surface* __restrict__ surf, const bt_medium* __restrict__ m2,
-    double reflect, double transmit, const bt_obsc* __restrict__ obsc, int olen, bt_rays rv,
-    bt_rays sp, int niter, int live, double min_flux)
+// One ray of rSplit; returns false for a ray traceSplit's filter had
+// already dropped (live mode), which is left untouched.  `lr` / `ls`: the
+// refracted and split parts pass traceSplit's filter afterwards.
+template <int K>
+__device__ __forceinline__ void rsplit_ray(
+    int64_t i, const Xform& xf, const bt_surface* __restrict__ surf,
+    const bt_medium* __restrict__ m2, double reflect, double transmit,
+    const bt_obsc* __restrict__ obsc, int olen, const bt_rays& rv, const bt_rays& sp,
+    int niter, int live, double min_flux, bool& lr, bool& ls)
 {                                                    // batoid.cpp:582-672
-    const int64_t i = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (i >= rv.n) return;
+    lr = ls = false;
     // traceSplit's filter (optic.py:1262) has already dropped this ray: it
     // stays as it is, and its split part is only marked dropped too
     if (live && (rv.vignetted[i] != 0 || !(rv.flux[i] >= min_flux))) {
@@ -42,8 +46,9 @@ __global__ __launch_bounds__(kBlock) void k_rsplit(
     // the split ray starts as a copy of the input (trace.py:125)
     double sx = x0, sy = y0, sz = z0, svx = vx0, svy = vy0, svz = vz0, st = t0, sf = f;
     bool svig = vig0, sfail = fail0;
-    // the refracted ray's final position and flag, for the obscuration
-    double rx = x0, ry = y0;
+    // the refracted ray's final position, flag and flux, for the obscuration
+    // and the filter
+    double rx = x0, ry = y0, rf = f;
     bool rvig = vig0;
     if (!fail0) {
         double dx = x0-dr[0];
@@ -59,7 +64,7 @@ __global__ __launch_bounds__(kBlock) void k_rsplit(
         double dt = 0.0;
         NormalMemo memo;
         memo.ok = false;
-        const bool hit = surf_tti(surf, x, y, z, vx, vy, vz, dt, niter, &memo);
+        const bool hit = surf_tti<K>(surf, x, y, z, vx, vy, vz, dt, niter, &memo);
         if (hit) {
             x += vx * dt;
             y += vy * dt;
@@ -76,7 +81,7 @@ __global__ __launch_bounds__(kBlock) void k_rsplit(
             if (memo.ok && same_bits(memo.dt, dt)) {  // Newton's last normal, same point
                 nx = memo.nx; ny = memo.ny; nz = memo.nz;
             } else {
-                surf_normal(surf, x, y, nx, ny, nz);
+                surf_normal<K>(surf, x, y, nx, ny, nz);
             }
             double alpha = nvx*nx;
             alpha += nvy*ny;
@@ -119,7 +124,8 @@ __global__ __launch_bounds__(kBlock) void k_rsplit(
                 rv.vy[i] = uy / n2;
                 rv.vz[i] = uz / n2;
             }
-            rv.flux[i] = f*transmit;
+            rf = f*transmit;
+            rv.flux[i] = rf;
         } else {
             rv.failed[i] = 1;
             rv.vignetted[i] = 1;
@@ -131,7 +137,10 @@ __global__ __launch_bounds__(kBlock) void k_rsplit(
     // the Interface's obscuration on both parts (optic.py:491-494), every
     // ray included, as batoid.cpp:151-172
     if (olen > 0) {
-        if (!rvig && obsc_contains(obsc, olen, rx, ry)) rv.vignetted[i] = 1;
+        if (!rvig && obsc_contains(obsc, olen, rx, ry)) {
+            rv.vignetted[i] = 1;
+            rvig = true;
+        }
         if (!svig) svig = obsc_contains(obsc, olen, sx, sy);
     }
     sp.x[i] = sx; sp.y[i] = sy; sp.z[i] = sz;
@@ -141,6 +150,35 @@ __global__ __launch_bounds__(kBlock) void k_rsplit(
     sp.flux[i] = sf;
     sp.vignetted[i] = svig;
     sp.failed[i] = sfail;
+    lr = !rvig && rf >= min_flux;
+    ls = !svig && sf >= min_flux;
+}
+
+// `counts` (live mode, may be null): the numbers of refracted and split
+// rays that pass traceSplit's filter, added up by wave ballots into
+// kCountSlots partial sums each (the wave's block picks the slot), so that
+// no one address takes every wave's atomic.
+constexpr int kCountSlots = 64;
+
+template <int K>
+__global__ __launch_bounds__(kBlock) void k_rsplit(
+    Xform xf, const bt_surface* __restrict__ surf, const bt_medium* __restrict__ m2,
+    double reflect, double transmit, const bt_obsc* __restrict__ obsc, int olen, bt_rays rv,
+    bt_rays sp, int niter, int live, double min_flux, unsigned long long* __restrict__ counts)
+{
+    const int64_t i = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    bool lr = false, ls = false;
+    if (i < rv.n)
+        rsplit_ray<K>(i, xf, surf, m2, reflect, transmit, obsc, olen, rv, sp, niter, live,
+                      min_flux, lr, ls);
+    if (counts) {
+        const uint64_t mr = __ballot(lr), ms = __ballot(ls);
+        if ((threadIdx.x & 63) == 0) {
+            const int slot = (int)(blockIdx.x % kCountSlots);
+            if (mr) atomicAdd(counts + slot, (unsigned long long)__popcll(mr));
+            if (ms) atomicAdd(counts + kCountSlots + slot, (unsigned long long)__popcll(ms));
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -254,7 +292,7 @@ namespace {
 int rsplit_launch(const bt_surface* surface, const double* dr, const double* drot,
                   const bt_medium* m2, const bt_coating* coating, const bt_obsc* obsc,
                   int obsc_len, bt_rays rays, bt_rays split, int niter, int live,
-                  double min_flux, void* stream, const char* name)
+                  double min_flux, int kinds, int64_t* counts, void* stream, const char* name)
 {
     const std::string n(name);
     if (!surface || !dr || !drot || !m2 || !coating)
@@ -267,10 +305,19 @@ int rsplit_launch(const bt_surface* surface, const double* dr, const double* dro
     if (rays.n == 0) return 0;
     if (!rays_ok(rays) || !rays_ok(split))
         return fail(BT_E_NULL, n + ": incomplete ray columns");
-    hipLaunchKernelGGL(k_rsplit, dim3(grid_for(rays.n)), dim3(kBlock), 0, as_stream(stream),
-                       make_xform(dr, drot), surface, m2, coating->reflectivity,
-                       coating->transmissivity, obsc, obsc_len, rays, split, niter, live,
-                       min_flux);
+    if (rays.n > (int64_t)0x7fffffff * kBlock)
+        return fail(BT_E_SIZE, n + ": too many rays");
+    auto* c = reinterpret_cast<unsigned long long*>(counts);
+    if (kinds & BT_PLAN_ANALYTIC)
+        hipLaunchKernelGGL(k_rsplit<kKindsAnalytic>, dim3(grid_for(rays.n)), dim3(kBlock), 0,
+                           as_stream(stream), make_xform(dr, drot), surface, m2,
+                           coating->reflectivity, coating->transmissivity, obsc, obsc_len, rays,
+                           split, niter, live, min_flux, c);
+    else
+        hipLaunchKernelGGL(k_rsplit<kKindsAll>, dim3(grid_for(rays.n)), dim3(kBlock), 0,
+                           as_stream(stream), make_xform(dr, drot), surface, m2,
+                           coating->reflectivity, coating->transmissivity, obsc, obsc_len, rays,
+                           split, niter, live, min_flux, c);
     return check_launch(name);
 }
 }  // namespace
@@ -280,16 +327,16 @@ int bt_rsplit(const bt_surface* surface, const double* dr, const double* drot,
               int obsc_len, bt_rays rays, bt_rays split, int niter, void* stream)
 {
     return rsplit_launch(surface, dr, drot, m2, coating, obsc, obsc_len, rays, split, niter, 0,
-                         0.0, stream, "bt_rsplit");
+                         0.0, 0, nullptr, stream, "bt_rsplit");
 }
 
 int bt_rsplit_live(const bt_surface* surface, const double* dr, const double* drot,
                    const bt_medium* m2, const bt_coating* coating, const bt_obsc* obsc,
                    int obsc_len, bt_rays rays, bt_rays split, double min_flux, int niter,
-                   void* stream)
+                   int kinds, int64_t* counts, void* stream)
 {
     return rsplit_launch(surface, dr, drot, m2, coating, obsc, obsc_len, rays, split, niter, 1,
-                         min_flux, stream, "bt_rsplit_live");
+                         min_flux, kinds, counts, stream, "bt_rsplit_live");
 }
 
 int64_t bt_select_workspace_bytes(int64_t n)

@@ -7,6 +7,7 @@ plan and traces every ray through the whole optic in one fused GPU kernel
 from functools import cached_property
 
 import numpy as np
+import torch
 
 from . import _lib, _device
 from .constants import globalCoordSys, vacuum
@@ -150,7 +151,8 @@ class Interface(Optic):
         if self.name not in unless:
             self.obscuration = None
 
-    def traceSplit(self, rv, minFlux=1e-3, reverse=False, _verbose=False, _filtered=False):
+    def traceSplit(self, rv, minFlux=1e-3, reverse=False, _verbose=False, _filtered=False,
+                   _counts=None):
         """Split rays at this Interface into those continuing forward and
         those sent back (optic.py:466-535); returns two lists of
         RayVectors (None entries where a direction gets no rays), each
@@ -159,20 +161,32 @@ class Interface(Optic):
         ``_filtered`` (CompoundOptic.traceSplit's own use): ``rv`` still
         holds the rays traceSplit's filter dropped after the previous item
         (vignetted, or flux < minFlux); they are carried along, untouched,
-        instead of being copied out."""
+        instead of being copied out.  ``_counts``: (device int64 array,
+        index) of two counts the split adds the numbers of refracted and
+        reflected rays passing the filter to, each spread over
+        ``_COUNT_SLOTS`` entries (the outputs' ``_live_slot``)."""
         if _verbose:
             s = "forward" if not reverse else "reverse"
             print("traceSplit {}       {:30s} flux = {:18.8f}   nphot = {:10d}".format(
                 s, self.name, *_live_stats(rv, minFlux if _filtered else None)))
         if self.skip:
+            rv._live_slot = rv._live_count = None
             return (None, rv) if reverse else (rv, None)
         # the obscuration applies to both directions, in the same pass
-        refracted, reflected = self.rSplit(rv, reverse=reverse, obscuration=self.obscuration,
-                                           _live_min_flux=minFlux if _filtered else None)
+        live = _filtered and isinstance(self, (RefractiveInterface, Detector))
+        live_counts = None
+        if live and _counts is not None:
+            base, off = _counts
+            live_counts = base[off * _COUNT_SLOTS:(off + 2) * _COUNT_SLOTS]
+        refracted, reflected = self.rSplit(
+            rv, reverse=reverse, obscuration=self.obscuration,
+            _live_min_flux=minFlux if _filtered else None, _live_counts=live_counts)
         path = list(getattr(rv, 'path', [])) + [self.name]
-        for r in (refracted, reflected):
+        for k, r in enumerate((refracted, reflected)):
             if r is not None:
                 r.path = list(path)
+                r._live_slot = (base, off + k) if live_counts is not None else None
+                r._live_count = None
         rForward, rReverse = [refracted], [reflected]
         if reverse:
             rForward, rReverse = rReverse, rForward
@@ -180,10 +194,17 @@ class Interface(Optic):
             rForward, rReverse = rReverse, rForward
         return rForward, rReverse
 
-    def _split_batch(self, inputs, minFlux, _verbose=False):
+    def _split_batch(self, inputs, minFlux, _verbose=False, _compact=True):
         """traceSplit of several (rays, reverse, filtered) inputs (no host
-        synchronisation: the launches queue on the stream)."""
-        return [self.traceSplit(rv, minFlux, rev, _verbose, filt) for rv, rev, filt in inputs]
+        synchronisation: the launches queue on the stream); the filtered
+        inputs' splits count their kept rays into one device array."""
+        counts = None
+        if any(filt for _, _, filt in inputs):
+            counts = torch.zeros(2 * _COUNT_SLOTS * len(inputs), dtype=torch.int64,
+                                 device=_device.device())
+        return [self.traceSplit(rv, minFlux, rev, _verbose, filt,
+                                (counts, 2 * j) if counts is not None else None)
+                for j, (rv, rev, filt) in enumerate(inputs)]
 
     @cached_property
     def R_outer(self):
@@ -250,7 +271,8 @@ class RefractiveInterface(Interface):
         self.forwardCoating = SimpleCoating(reflectivity=0.0, transmissivity=1.0)
         self.reverseCoating = SimpleCoating(reflectivity=0.0, transmissivity=1.0)
 
-    def rSplit(self, rv, reverse=False, obscuration=None, _live_min_flux=None):
+    def rSplit(self, rv, reverse=False, obscuration=None, _live_min_flux=None,
+               _live_counts=None):
         """(refracted, reflected); as optic.py:786-797, the forward direction
         uses reverseCoating and the reverse direction forwardCoating."""
         if reverse:
@@ -258,7 +280,8 @@ class RefractiveInterface(Interface):
         else:
             m1, m2, coating = self.inMedium, self.outMedium, self.reverseCoating
         return self.surface.rSplit(rv, m1, m2, coating, coordSys=self.coordSys,
-                                   obscuration=obscuration, _live_min_flux=_live_min_flux)
+                                   obscuration=obscuration, _live_min_flux=_live_min_flux,
+                                   _live_counts=_live_counts)
 
     def _interact(self, rv, reverse, obscuration):
         from .trace import refract
@@ -277,7 +300,8 @@ class Mirror(Interface):
         self.forwardCoating = SimpleCoating(reflectivity=1.0, transmissivity=0.0)
         self.reverseCoating = SimpleCoating(reflectivity=1.0, transmissivity=0.0)
 
-    def rSplit(self, rv, reverse=False, obscuration=None, _live_min_flux=None):
+    def rSplit(self, rv, reverse=False, obscuration=None, _live_min_flux=None,
+               _live_counts=None):
         from .trace import reflect
         return None, reflect(self.surface, rv, self.coordSys, obscuration=obscuration)
 
@@ -297,11 +321,12 @@ class Detector(Interface):
         self.forwardCoating = SimpleCoating(reflectivity=0.0, transmissivity=1.0)
         self.reverseCoating = None
 
-    def rSplit(self, rv, reverse=False, obscuration=None, _live_min_flux=None):
+    def rSplit(self, rv, reverse=False, obscuration=None, _live_min_flux=None,
+               _live_counts=None):
         assert not reverse
         return self.surface.rSplit(rv, self.inMedium, self.outMedium, self.forwardCoating,
                                    coordSys=self.coordSys, obscuration=obscuration,
-                                   _live_min_flux=_live_min_flux)
+                                   _live_min_flux=_live_min_flux, _live_counts=_live_counts)
 
 
 class Baffle(Interface):
@@ -312,7 +337,8 @@ class Baffle(Interface):
         self.forwardCoating = SimpleCoating(reflectivity=0.0, transmissivity=1.0)
         self.reverseCoating = SimpleCoating(reflectivity=0.0, transmissivity=1.0)
 
-    def rSplit(self, rv, reverse=False, obscuration=None, _live_min_flux=None):
+    def rSplit(self, rv, reverse=False, obscuration=None, _live_min_flux=None,
+               _live_counts=None):
         from .trace import intersect
         return intersect(self.surface, rv, self.coordSys, obscuration=obscuration), None
 
@@ -328,8 +354,56 @@ class OPDScreen(Interface):
     def _interact(self, rv, reverse, obscuration):
         raise NotImplementedError("OPDScreen is outside the GPU trace path")
 
-    def rSplit(self, rv, reverse=False, obscuration=None, _live_min_flux=None):
+    def rSplit(self, rv, reverse=False, obscuration=None, _live_min_flux=None,
+               _live_counts=None):
         raise NotImplementedError("OPDScreen is outside the GPU trace path")
+
+
+# partial sums per count that bt_rsplit_live spreads its atomics over
+_COUNT_SLOTS = 64
+
+
+def _count_children(children, minFlux, compact):
+    """The number of rays of each child set that pass traceSplit's filter,
+    read back in one synchronisation: from the split kernel's own counts
+    (``_live_slot``), from a nested optic (``_live_count``), or counted by
+    bt_select_count -- always so for sets that are to be copied out
+    (leaving the optic with ``compact``), whose copy needs that pass's
+    offsets.  Returns (counts, copy), copy(i, m) giving child i's kept rays
+    as a new RayVector."""
+    need, slot_of = [], {}
+    bases = []
+    for ci, (_, _, rr, leaves) in enumerate(children):
+        if leaves and compact:
+            need.append(ci)
+        elif getattr(rr, '_live_count', None) is not None:
+            continue
+        elif getattr(rr, '_live_slot', None) is not None:
+            base = rr._live_slot[0]
+            if id(base) not in slot_of:
+                slot_of[id(base)] = len(bases)
+                bases.append(base)
+        else:
+            need.append(ci)
+    sel = RayVector._count_select([children[ci][2] for ci in need], minFlux) if need else None
+    parts = [b.view(-1, _COUNT_SLOTS).sum(1) for b in bases]
+    parts += [sel.counts] if sel is not None else []
+    vals = torch.cat(parts).tolist() if parts else []
+    starts, pos = [], 0
+    for b in bases:
+        starts.append(pos)
+        pos += b.numel() // _COUNT_SLOTS
+    need_at = {ci: j for j, ci in enumerate(need)}
+    counts = []
+    for ci, (_, _, rr, leaves) in enumerate(children):
+        if ci in need_at:
+            counts.append(int(vals[pos + need_at[ci]]))
+        elif getattr(rr, '_live_count', None) is not None:
+            counts.append(int(rr._live_count))
+        else:
+            base, off = rr._live_slot
+            counts.append(int(vals[starts[slot_of[id(base)]] + off]))
+    return counts, (lambda ci, m: sel.scatter(need_at[ci], m))
 
 
 def _live_stats(rv, minFlux):
@@ -517,8 +591,8 @@ class CompoundOptic(Optic):
                 s, self.name, *_live_stats(rv, None)))
         return self._split_batch([(rv, reverse, False)], minFlux, _verbose)[0]
 
-    def _split_batch(self, inputs, minFlux, _verbose=False):
-        """traceSplit of several (rays, reverse) inputs at once.
+    def _split_batch(self, inputs, minFlux, _verbose=False, _compact=True):
+        """traceSplit of several (rays, reverse, filtered) inputs at once.
 
         The reference pops one work item at a time from a LIFO stack and
         reads back the counts of its outputs before the next.  Here every
@@ -532,12 +606,16 @@ class CompoundOptic(Optic):
         reproduces the reference's lists exactly.
 
         The filter itself (unvignetted, flux >= minFlux) is only counted
-        for the ray sets that go on to another item: they keep the dropped
-        rays, which the next items skip (``_filtered``); only the sets that
-        leave the optic are copied out, kept rays in order.  Each returned
-        set therefore holds exactly the rays the reference's per-item
-        copies would."""
+        for the ray sets that go on to another item (by the split kernel
+        itself where it can): they keep the dropped rays, which the next
+        items skip (``_filtered``).  Only the sets that leave the outermost
+        optic are copied out, kept rays in order (``_compact``; a nested
+        optic returns its sets uncopied, with their counts), so each
+        returned set holds exactly the rays the reference's per-item copies
+        would."""
         if self.skip:
+            for rv, _, _ in inputs:
+                rv._live_slot = rv._live_count = None
             return [((None, rv) if rev else (rv, None)) for rv, rev, _ in inputs]
         last = len(self.items) - 1
         # work entries: (input, key, rays, reverse, item index, filtered)
@@ -553,26 +631,30 @@ class CompoundOptic(Optic):
             for index in sorted(groups):
                 entries = groups[index]
                 results = self.items[index]._split_batch(
-                    [(w[2], w[3], w[5]) for w in entries], minFlux, _verbose)
+                    [(w[2], w[3], w[5]) for w in entries], minFlux, _verbose, _compact=False)
                 for w, (rF, rR) in zip(entries, results):
                     for side, rl in enumerate((rF, rR)):
                         if not isinstance(rl, list):   # a skipped item passes rays on
                             rl = [rl]
                         leaves = (side == 0 and w[4] == last) or (side == 1 and w[4] == 0)
                         children += [(w, side, rr, leaves) for rr in rl if rr]
-            sels = RayVector._select_many([c[2] for c in children], minFlux,
-                                          compact=[c[3] for c in children])
+            counts, copy = _count_children(children, minFlux, _compact)
             pushed, seq = {}, {}
-            for (w, side, rr, leaves), (sel, m) in zip(children, sels):
+            for ci, ((w, side, rr, leaves), m) in enumerate(zip(children, counts)):
                 if m == 0:
                     continue
                 k, key, _, _, index, _ = w
                 parent = id(w)
                 if leaves:
-                    sel.path = rr.path
+                    if _compact:
+                        out = copy(ci, m)
+                        out.path = rr.path
+                    else:
+                        out = rr
+                        out._live_slot, out._live_count = None, m
                     s = seq.get(parent, 0)
                     seq[parent] = s + 1
-                    outs[k][side].append(((key, s), sel))
+                    outs[k][side].append(((key, s), out))
                 else:
                     p = pushed.get(parent, 0)
                     pushed[parent] = p + 1

@@ -136,39 +136,21 @@ class RayVector:
         return out
 
     @staticmethod
-    def _select_many(rvs, minFlux, compact=None):
-        """``[(rv._select(minFlux), count) for rv in rvs]`` with one host
-        synchronisation for all the counts instead of one per RayVector.
-        Where ``compact[i]`` is false only the count is taken: that entry
-        is ``(None, count)``."""
+    def _count_select(rvs, minFlux):
+        """bt_select_count over several RayVectors into one device array
+        (``.counts``, no host synchronisation); ``.scatter(i, m)`` then
+        copies out the m kept rays of rvs[i] (the filter of
+        optic.py:1256-1266)."""
+        return _SelectCounts(rvs, minFlux)
+
+    @staticmethod
+    def _select_many(rvs, minFlux):
+        """``[rv._select(minFlux) for rv in rvs]`` with one host
+        synchronisation for all the counts instead of one per RayVector."""
         if not rvs:
             return []
-        if compact is None:
-            compact = [True] * len(rvs)
-        lib = _lib.load()
-        stream = _device.stream()
-        sizes = [int(lib.bt_select_workspace_bytes(len(rv))) for rv in rvs]
-        offs, tot = [], 0
-        for b in sizes:
-            offs.append(tot)
-            tot += -(-b // 256) * 256
-        ws = torch.empty(tot + 8 * len(rvs), dtype=torch.uint8, device=_device.device())
-        counts = ws[tot:].view(torch.int64)            # written by bt_select_count
-        for i, rv in enumerate(rvs):
-            _lib.call("bt_select_count", rv._rays, float(minFlux), _device.ptr(counts[i:]),
-                      _device.ptr(ws[offs[i]:]), sizes[i], stream)
-        ms = counts.tolist()
-        outs = []
-        for i, rv in enumerate(rvs):
-            if not compact[i]:
-                outs.append((None, ms[i]))
-                continue
-            out = RayVector._empty(ms[i], rv.coordSys)
-            if ms[i]:
-                _lib.call("bt_select_scatter", rv._rays, float(minFlux), out._rays,
-                          _device.ptr(ws[offs[i]:]), sizes[i], stream)
-            outs.append((out, ms[i]))
-        return outs
+        sel = _SelectCounts(rvs, minFlux)
+        return [sel.scatter(i, m) for i, m in enumerate(sel.counts.tolist())]
 
     @classmethod
     def fromTorch(cls, x, y, z, vx, vy, vz, t, wavelength, flux, vignetted, failed,
@@ -635,3 +617,32 @@ def concatenateRayVectors(rvs):
     """Rays of several RayVectors in one (rayVector.py:1351-1365)."""
     cols = {k: torch.cat([rv._cols[k] for rv in rvs]) for k in _COLS}
     return RayVector._directInit(cols, rvs[0].coordSys)
+
+
+class _SelectCounts:
+    """bt_select_count launched over several ray sets, their counts in one
+    device int64 array and their per-block offsets kept for the copies."""
+
+    def __init__(self, rvs, minFlux):
+        lib = _lib.load()
+        self.rvs = rvs
+        self.minFlux = float(minFlux)
+        self.sizes = [int(lib.bt_select_workspace_bytes(len(rv))) for rv in rvs]
+        self.offs, tot = [], 0
+        for b in self.sizes:
+            self.offs.append(tot)
+            tot += -(-b // 256) * 256
+        self.ws = torch.empty(tot + 8 * len(rvs), dtype=torch.uint8, device=_device.device())
+        self.counts = self.ws[tot:].view(torch.int64)   # written by bt_select_count
+        stream = _device.stream()
+        for i, rv in enumerate(rvs):
+            _lib.call("bt_select_count", rv._rays, self.minFlux, _device.ptr(self.counts[i:]),
+                      _device.ptr(self.ws[self.offs[i]:]), self.sizes[i], stream)
+
+    def scatter(self, i, m):
+        rv = self.rvs[i]
+        out = RayVector._empty(m, rv.coordSys)
+        if m:
+            _lib.call("bt_select_scatter", rv._rays, self.minFlux, out._rays,
+                      _device.ptr(self.ws[self.offs[i]:]), self.sizes[i], _device.stream())
+        return out

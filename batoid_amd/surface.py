@@ -93,12 +93,22 @@ class Surface(DeviceCached):
         return reflect(self, rv, coordSys, coating)
 
     def rSplit(self, rv, inMedium, outMedium, coating, coordSys=None, obscuration=None,
-               _live_min_flux=None):
+               _live_min_flux=None, _live_counts=None):
         """Refracted (``rv``, in place) and reflected (new) rays at this
         surface, fluxes split by ``coating`` (surface.py:107-133)."""
         from .trace import rSplit
         return rSplit(self, rv, inMedium, outMedium, coating, coordSys, obscuration,
-                      _live_min_flux)
+                      _live_min_flux, _live_counts)
+
+    @property
+    def _analytic(self):
+        """Every record of this surface is a Plane, Sphere, Paraboloid,
+        Quadric, Asphere or Tilted."""
+        a = self.__dict__.get("_analytic_c")
+        if a is None:
+            a = all(r.kind in _lib.ANALYTIC_KINDS for r in self._records()[0])
+            self.__dict__["_analytic_c"] = a
+        return a
 
     def refract(self, rv, inMedium, outMedium, coordSys=None, coating=None):
         """Intersect and refract rays in place (trace.py:101-116)."""

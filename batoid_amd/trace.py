@@ -112,7 +112,7 @@ def refract(surface, rv, m1, m2, coordSys=None, coating=None, obscuration=None):
 
 
 def rSplit(surface, rv, inMedium, outMedium, coating, coordSys=None, obscuration=None,
-           _live_min_flux=None):
+           _live_min_flux=None, _live_counts=None):
     """Split rays at ``surface`` into refracted (``rv``, in place) and
     reflected (a new RayVector) parts with the coating's flux fractions
     (trace.py:119-136, batoid.cpp:509-674).  ``obscuration``, if given, is
@@ -120,7 +120,9 @@ def rSplit(surface, rv, inMedium, outMedium, coating, coordSys=None, obscuration
 
     ``_live_min_flux`` (traceSplit's own use): ``rv`` still holds rays that
     traceSplit's filter dropped (vignetted, or flux below this threshold);
-    they are skipped and only marked dropped in the reflected part."""
+    they are skipped and only marked dropped in the reflected part.
+    ``_live_counts``: a device int64[2] the numbers of refracted and
+    reflected rays passing that filter are added to."""
     from .rayVector import RayVector
     if coordSys is None:
         coordSys = rv.coordSys
@@ -132,7 +134,10 @@ def rSplit(surface, rv, inMedium, outMedium, coating, coordSys=None, obscuration
     if _live_min_flux is None:
         _lib.call("bt_rsplit", *args, _niter(), _device.stream())
     else:
-        _lib.call("bt_rsplit_live", *args, float(_live_min_flux), _niter(), _device.stream())
+        kinds = _lib.PLAN_ANALYTIC if surface._analytic else 0
+        cptr = _device.ptr(_live_counts) if _live_counts is not None else None
+        _lib.call("bt_rsplit_live", *args, float(_live_min_flux), _niter(), kinds, cptr,
+                  _device.stream())
     rv.coordSys = coordSys
     return rv, split
 

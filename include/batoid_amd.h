@@ -376,12 +376,19 @@ int bt_rsplit(const bt_surface* surface, const double* dr, const double* drot,
  * only their split part is marked vignetted (its other columns are not
  * written); every other ray is split exactly as by bt_rsplit.  Filtering
  * the two outputs afterwards gives the rays bt_rsplit on the filtered set
- * and the same filter would give.  The reference has no such call: its
- * traceSplit copies out the kept rays after every Interface.              */
+ * and the same filter would give.  `kinds`: BT_PLAN_ANALYTIC when the
+ * surface record is a Plane, Sphere, Paraboloid, Quadric, Asphere or Tilted
+ * (a kernel without the other kinds' code; under it any other kind fails
+ * its rays), else 0.  `counts` (a device int64[2 * 64], or NULL) is
+ * incremented by the numbers of refracted (entries 0..63) and split
+ * (entries 64..127) rays that pass the filter, each number spread over its
+ * 64 entries: their sum is the count.  The
+ * reference has no such call: its traceSplit copies out the kept rays
+ * after every Interface.                                                  */
 int bt_rsplit_live(const bt_surface* surface, const double* dr, const double* drot,
                    const bt_medium* m2, const bt_coating* coating, const bt_obsc* obsc,
                    int obsc_len, bt_rays rays, bt_rays split, double min_flux, int niter,
-                   void* stream);
+                   int kinds, int64_t* counts, void* stream);
 
 /* ---- ray selection ------------------------------------------------------
  * The ray filter of CompoundOptic.traceSplit (optic.py:1256-1266): keep the

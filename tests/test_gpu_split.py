@@ -108,7 +108,9 @@ def test_rsplit_live_skips_dropped_rays(name, surf):
     """bt_rsplit_live on a set that still holds rays traceSplit's filter
     dropped (vignetted, flux below minFlux, NaN flux): those rays are left
     exactly as they were and their split part is marked vignetted; every
-    kept ray is split bit for bit as bt_rsplit splits it."""
+    kept ray is split bit for bit as bt_rsplit splits it (the analytic-kinds
+    kernel for the sphere and the asphere, the all-kinds one for the Sum),
+    and the kernel's counts of kept output rays are the filter's."""
     med = media()[2][1]
     coat = bt.SimpleCoating(0.0371, 0.9588)
     obsc = bt.ObscNegation(bt.ObscAnnulus(0.2, 0.8))
@@ -125,9 +127,14 @@ def test_rsplit_live_skips_dropped_rays(name, surf):
         assert 0 < keep.sum() < N
         full_r, full_s = surf.rSplit(_gpu(base), bt.ConstMedium(1.0), med, coat, coordSys=cs,
                                      obscuration=obsc)
+        counts = torch.zeros(2 * 64, dtype=torch.int64, device="cuda")
         live_r, live_s = surf.rSplit(_gpu(base), bt.ConstMedium(1.0), med, coat, coordSys=cs,
-                                     obscuration=obsc, _live_min_flux=minFlux)
+                                     obscuration=obsc, _live_min_flux=minFlux,
+                                     _live_counts=counts)
         fr, fs, lr, ls = (_host(r) for r in (full_r, full_s, live_r, live_s))
+        # the kernel's own counts of the rays each output keeps
+        for c, h in zip(counts.view(2, 64).sum(1).tolist(), (fr, fs)):
+            assert c == int(((h["vignetted"] == 0) & (h["flux"] >= minFlux) & keep).sum())
         for k in FCOLS + BCOLS:
             assert_cols_identical({k: lr[k][keep]}, {k: fr[k][keep]}, cols=(k,),
                                   what=f"{name}/{tname} kept refracted")
