@@ -144,6 +144,7 @@ SIGNATURES = {
     "bt_refract": (_I, [_V, _V, _V, _V, BtRays, _I, _V, _V, _I, _V]),
     "bt_rsplit": (_I, [_V, _V, _V, _V, _V, _V, _I, BtRays, BtRays, _I, _V]),
     "bt_rsplit_live": (_I, [_V, _V, _V, _V, _V, _V, _I, BtRays, BtRays, _D, _I, _I, _V, _V]),
+    "bt_interact_live": (_I, [_I, _V, _V, _V, _V, BtRays, _I, _V, _V, _I, _D, _I, _V, _V]),
     "bt_select_workspace_bytes": (_L, [_L]),
     "bt_select_count": (_I, [BtRays, _D, _V, _V, _L, _V]),
     "bt_select_scatter": (_I, [BtRays, _D, BtRays, _V, _L, _V]),

@@ -136,16 +136,25 @@ __global__ __launch_bounds__(kBlock) void k_finish_parallel(
 // One Interface over all rays, in place.  coat_on / coat: the coating's
 // flux factor for rays that reach the surface (batoid.cpp:251-263, 362-371,
 // 497-499); a SimpleCoating's factor does not depend on the incidence.
-__global__ __launch_bounds__(kBlock) void k_step(
-    int action, Xform xf, const bt_surface* __restrict__ surf,
-    const bt_medium* __restrict__ m2, const bt_obsc* __restrict__ obsc,
-    int obsc_len, bt_rays rv, int niter, int coat_on, double coat)
+// live / min_flux / counts: as k_rsplit's (traceSplit's carried sets): a
+// ray the filter already dropped is skipped, and the kept rays of the
+// output are counted into kStepCountSlots partial sums.
+constexpr int kStepCountSlots = 64;
+
+template <int K>
+__device__ __forceinline__ bool step_ray(
+    int64_t i, int action, const Xform& xf, const bt_surface* __restrict__ surf,
+    const bt_medium* __restrict__ m2, const bt_obsc* __restrict__ obsc, int obsc_len,
+    const bt_rays& rv, int niter, int coat_on, double coat, int live, double min_flux)
 {
-    const int64_t i = ray_index();
-    if (i >= rv.n) return;
     Ray r;
     r.fail = rv.failed[i] != 0;
     r.vig = rv.vignetted[i] != 0;
+    double f = 0.0;
+    if (live) {
+        f = rv.flux[i];
+        if (r.vig || !(f >= min_flux)) return false;
+    }
     const bool fail0 = r.fail, vig0 = r.vig;
     r.x = rv.x[i]; r.y = rv.y[i]; r.z = rv.z[i];
     r.vx = rv.vx[i]; r.vy = rv.vy[i]; r.vz = rv.vz[i];
@@ -155,17 +164,42 @@ __global__ __launch_bounds__(kBlock) void k_step(
         r.w = rv.wavelength[i];
         n2 = medium_n(m2, r.w);
     }
-    const bool moved = apply_interaction(action, xf.dr, xf.drot, surf, n2, r, niter);
+    const bool moved = apply_interaction<K>(action, xf.dr, xf.drot, surf, n2, r, niter);
     if (moved) {
         rv.x[i] = r.x; rv.y[i] = r.y; rv.z[i] = r.z;
         rv.vx[i] = r.vx; rv.vy[i] = r.vy; rv.vz[i] = r.vz;
         rv.t[i] = r.t;
-        if (coat_on) rv.flux[i] *= coat;
+        if (coat_on) {
+            if (!live) f = rv.flux[i];
+            f *= coat;
+            rv.flux[i] = f;
+        }
     }
     if (obsc_len > 0 && !r.vig)                     // batoid.cpp:169-171
         r.vig = obsc_contains(obsc, obsc_len, r.x, r.y);
     if (r.vig != vig0) rv.vignetted[i] = r.vig;
     if (r.fail != fail0) rv.failed[i] = r.fail;
+    return live && !r.vig && f >= min_flux;
+}
+
+template <int K>
+__global__ __launch_bounds__(kBlock) void k_step(
+    int action, Xform xf, const bt_surface* __restrict__ surf,
+    const bt_medium* __restrict__ m2, const bt_obsc* __restrict__ obsc,
+    int obsc_len, bt_rays rv, int niter, int coat_on, double coat, int live,
+    double min_flux, unsigned long long* __restrict__ counts)
+{
+    const int64_t i = ray_index();
+    bool keep = false;
+    if (i < rv.n)
+        keep = step_ray<K>(i, action, xf, surf, m2, obsc, obsc_len, rv, niter, coat_on, coat,
+                           live, min_flux);
+    if (counts) {
+        const uint64_t m = __ballot(keep);
+        if ((threadIdx.x & 63) == 0 && m)
+            atomicAdd(counts + (int)(blockIdx.x % kStepCountSlots),
+                      (unsigned long long)__popcll(m));
+    }
 }
 
 __global__ __launch_bounds__(kBlock) void k_obscure(
@@ -378,7 +412,8 @@ bool valid_surface_kind(int k) { return k >= BT_PLANE && k <= BT_SUM; }
 int step_launch(int action, const bt_surface* surface, const double* dr,
                 const double* drot, const bt_medium* m2, bt_rays rays,
                 int niter, const bt_coating* coating, const bt_obsc* obsc,
-                int obsc_len, void* stream, const char* name)
+                int obsc_len, void* stream, const char* name, int live = 0,
+                double min_flux = 0.0, int kinds = 0, int64_t* counts = nullptr)
 {
     if (!surface || !dr || !drot) return fail(BT_E_NULL, std::string(name) + ": null argument");
     if (!rays_ok(rays)) return fail(BT_E_NULL, std::string(name) + ": incomplete ray columns");
@@ -390,9 +425,15 @@ int step_launch(int action, const bt_surface* surface, const double* dr,
     const double coat = !coating ? 1.0
                         : (action == BT_ACT_REFLECT ? coating->reflectivity
                                                     : coating->transmissivity);
-    hipLaunchKernelGGL(k_step, dim3(grid_for(rays.n)), dim3(kBlock), 0, as_stream(stream),
-                       action, make_xform(dr, drot), surface, m2, obsc, obsc_len, rays, niter,
-                       coating ? 1 : 0, coat);
+    auto* c = reinterpret_cast<unsigned long long*>(counts);
+    if (kinds & BT_PLAN_ANALYTIC)
+        hipLaunchKernelGGL(k_step<kKindsAnalytic>, dim3(grid_for(rays.n)), dim3(kBlock), 0,
+                           as_stream(stream), action, make_xform(dr, drot), surface, m2, obsc,
+                           obsc_len, rays, niter, coating ? 1 : 0, coat, live, min_flux, c);
+    else
+        hipLaunchKernelGGL(k_step<kKindsAll>, dim3(grid_for(rays.n)), dim3(kBlock), 0,
+                           as_stream(stream), action, make_xform(dr, drot), surface, m2, obsc,
+                           obsc_len, rays, niter, coating ? 1 : 0, coat, live, min_flux, c);
     return check_launch(name);
 }
 
@@ -589,6 +630,17 @@ int bt_refract(const bt_surface* surface, const double* dr,
 {
     return step_launch(BT_ACT_REFRACT, surface, dr, drot, m2, rays, niter,
                        coating, obsc, obsc_len, stream, "bt_refract");
+}
+
+int bt_interact_live(int action, const bt_surface* surface, const double* dr,
+                     const double* drot, const bt_medium* m2, bt_rays rays, int niter,
+                     const bt_coating* coating, const bt_obsc* obsc, int obsc_len,
+                     double min_flux, int kinds, int64_t* counts, void* stream)
+{
+    if (action < BT_ACT_INTERSECT || action > BT_ACT_REFRACT)
+        return fail(BT_E_SIZE, "bt_interact_live: unknown action");
+    return step_launch(action, surface, dr, drot, m2, rays, niter, coating, obsc, obsc_len,
+                       stream, "bt_interact_live", 1, min_flux, kinds, counts);
 }
 
 int bt_obscure(const bt_obsc* obsc, int obsc_len, bt_rays rays, void* stream)

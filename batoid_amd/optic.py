@@ -173,7 +173,7 @@ class Interface(Optic):
             rv._live_slot = rv._live_count = None
             return (None, rv) if reverse else (rv, None)
         # the obscuration applies to both directions, in the same pass
-        live = _filtered and isinstance(self, (RefractiveInterface, Detector))
+        live = _filtered and isinstance(self, (RefractiveInterface, Detector, Mirror, Baffle))
         live_counts = None
         if live and _counts is not None:
             base, off = _counts
@@ -303,7 +303,9 @@ class Mirror(Interface):
     def rSplit(self, rv, reverse=False, obscuration=None, _live_min_flux=None,
                _live_counts=None):
         from .trace import reflect
-        return None, reflect(self.surface, rv, self.coordSys, obscuration=obscuration)
+        counts = _live_counts[_COUNT_SLOTS:] if _live_counts is not None else None
+        return None, reflect(self.surface, rv, self.coordSys, obscuration=obscuration,
+                             _live_min_flux=_live_min_flux, _live_counts=counts)
 
     def _interact(self, rv, reverse, obscuration):
         from .trace import reflect
@@ -340,7 +342,9 @@ class Baffle(Interface):
     def rSplit(self, rv, reverse=False, obscuration=None, _live_min_flux=None,
                _live_counts=None):
         from .trace import intersect
-        return intersect(self.surface, rv, self.coordSys, obscuration=obscuration), None
+        counts = _live_counts[:_COUNT_SLOTS] if _live_counts is not None else None
+        return intersect(self.surface, rv, self.coordSys, obscuration=obscuration,
+                         _live_min_flux=_live_min_flux, _live_counts=counts), None
 
 
 class OPDScreen(Interface):

@@ -65,32 +65,58 @@ def _coating_ptr(coating):
     return ctypes.byref(coating._record)
 
 
-def intersect(surface, rv, coordSys=None, coating=None, obscuration=None):
+def _interact_live(action, surface, rv, ct, m2, coating, obscuration, live_min_flux,
+                   live_counts):
+    """bt_interact_live: the interaction for traceSplit's carried sets (rays
+    its filter dropped are skipped; the kept rays of the result are counted
+    into ``live_counts``, 64 partial sums, if given)."""
+    optr, olen = _obsc_args(obscuration)
+    kinds = _lib.PLAN_ANALYTIC if surface._analytic else 0
+    cptr = _device.ptr(live_counts) if live_counts is not None else None
+    _lib.call("bt_interact_live", action, surface._dev.ptr(), _lib.dvec(ct.dr),
+              _lib.dvec(ct.drot.ravel()), m2._dev.ptr() if m2 is not None else None,
+              rv._rays, _niter(), _coating_ptr(coating), optr, olen, float(live_min_flux),
+              kinds, cptr, _device.stream())
+
+
+def intersect(surface, rv, coordSys=None, coating=None, obscuration=None,
+              _live_min_flux=None, _live_counts=None):
     """Propagate rays to ``surface`` (in ``coordSys``) in place
     (trace.py:47-81); a coating scales the flux of the rays that reach it by
     its transmissivity.  ``obscuration``, if given, is applied in the same
-    pass."""
+    pass.  ``_live_min_flux`` / ``_live_counts``: traceSplit's carried sets,
+    as for rSplit."""
     if coordSys is None:
         coordSys = rv.coordSys
     ct = CoordTransform(rv.coordSys, coordSys)
-    optr, olen = _obsc_args(obscuration)
-    _lib.call("bt_intersect", surface._dev.ptr(), _lib.dvec(ct.dr),
-              _lib.dvec(ct.drot.ravel()), rv._rays, _niter(), _coating_ptr(coating), optr,
-              olen, _device.stream())
+    if _live_min_flux is not None:
+        _interact_live(_lib.ACT_INTERSECT, surface, rv, ct, None, coating, obscuration,
+                       _live_min_flux, _live_counts)
+    else:
+        optr, olen = _obsc_args(obscuration)
+        _lib.call("bt_intersect", surface._dev.ptr(), _lib.dvec(ct.dr),
+                  _lib.dvec(ct.drot.ravel()), rv._rays, _niter(), _coating_ptr(coating), optr,
+                  olen, _device.stream())
     rv.coordSys = coordSys
     return rv
 
 
-def reflect(surface, rv, coordSys=None, coating=None, obscuration=None):
+def reflect(surface, rv, coordSys=None, coating=None, obscuration=None,
+            _live_min_flux=None, _live_counts=None):
     """Intersect and reflect rays in place (trace.py:84-98); a coating scales
-    the flux by its reflectivity."""
+    the flux by its reflectivity.  ``_live_min_flux`` / ``_live_counts``:
+    traceSplit's carried sets, as for rSplit."""
     if coordSys is None:
         coordSys = rv.coordSys
     ct = CoordTransform(rv.coordSys, coordSys)
-    optr, olen = _obsc_args(obscuration)
-    _lib.call("bt_reflect", surface._dev.ptr(), _lib.dvec(ct.dr),
-              _lib.dvec(ct.drot.ravel()), rv._rays, _niter(), _coating_ptr(coating), optr,
-              olen, _device.stream())
+    if _live_min_flux is not None:
+        _interact_live(_lib.ACT_REFLECT, surface, rv, ct, None, coating, obscuration,
+                       _live_min_flux, _live_counts)
+    else:
+        optr, olen = _obsc_args(obscuration)
+        _lib.call("bt_reflect", surface._dev.ptr(), _lib.dvec(ct.dr),
+                  _lib.dvec(ct.drot.ravel()), rv._rays, _niter(), _coating_ptr(coating), optr,
+                  olen, _device.stream())
     rv.coordSys = coordSys
     return rv
 

@@ -357,6 +357,18 @@ int bt_refract(const bt_surface* surface, const double* dr,
                const double* drot, const bt_medium* m2, bt_rays rays,
                int niter, const bt_coating* coating, const bt_obsc* obsc,
                int obsc_len, void* stream);
+/* bt_intersect (BT_ACT_INTERSECT), bt_reflect (BT_ACT_REFLECT) or
+ * bt_refract (BT_ACT_REFRACT, m2 used) for ray sets that traceSplit's
+ * filter (optic.py:1256-1266) has been applied to without removing the
+ * rays it drops: rays with vignetted != 0 or flux < min_flux (NaN
+ * included) are left unchanged; every other ray is processed exactly as
+ * by the plain call.  `kinds` and `counts` as for bt_rsplit_live, with one
+ * count spread over counts[0..63].  No reference counterpart (see
+ * bt_rsplit_live).                                                        */
+int bt_interact_live(int action, const bt_surface* surface, const double* dr,
+                     const double* drot, const bt_medium* m2, bt_rays rays, int niter,
+                     const bt_coating* coating, const bt_obsc* obsc, int obsc_len,
+                     double min_flux, int kinds, int64_t* counts, void* stream);
 /* Replaces _batoid.rSplit (src/batoid.cpp:509-674, pysrc/batoid.cpp:58):
  * each ray that reaches the surface is split into the refracted ray
  * (written back to `rays`, flux x transmissivity) and the reflected ray

@@ -155,6 +155,40 @@ def test_rsplit_live_skips_dropped_rays(name, surf):
                                   what=f"{name}/{tname} filtered outputs")
 
 
+@pytest.mark.parametrize("action", ["intersect", "reflect"])
+@pytest.mark.parametrize("name,surf", [surfaces()[i] for i in (2, 9, 15)],
+                         ids=[surfaces()[i][0] for i in (2, 9, 15)])
+def test_interact_live_skips_dropped_rays(name, surf, action):
+    """bt_interact_live (traceSplit's Mirror and Baffle steps on carried
+    sets): dropped rays untouched, kept rays bit-identical to the plain
+    call, and the kernel's count of kept output rays the filter's."""
+    from batoid_amd import trace as T
+    coat = bt.SimpleCoating(0.3, 0.7)
+    obsc = bt.ObscNegation(bt.ObscAnnulus(0.2, 0.8))
+    rng = np.random.default_rng(31)
+    minFlux = 1e-3
+    fn = getattr(T, action)
+    for tname, (dr, drot) in transforms():
+        cs = _frame(dr, drot)
+        base = HostRays(**random_rays(N, seed=33, upward=0.3, spread=0.3))
+        flux = rng.uniform(0.0, 3e-3, N)
+        flux[rng.uniform(size=N) < 0.02] = np.nan
+        base.cols["flux"] = flux
+        base.cols["vignetted"] = (rng.uniform(size=N) < 0.2).astype(np.uint8)
+        keep = (base["vignetted"] == 0) & (base["flux"] >= minFlux)
+        full = _host(fn(surf, _gpu(base), cs, coating=coat, obscuration=obsc))
+        counts = torch.zeros(64, dtype=torch.int64, device="cuda")
+        live = _host(fn(surf, _gpu(base), cs, coating=coat, obscuration=obsc,
+                        _live_min_flux=minFlux, _live_counts=counts))
+        for k in FCOLS + BCOLS:
+            assert_cols_identical({k: live[k][keep]}, {k: full[k][keep]}, cols=(k,),
+                                  what=f"{name}/{action}/{tname} kept rays")
+            assert_cols_identical({k: live[k][~keep]}, {k: base[k][~keep]}, cols=(k,),
+                                  what=f"{name}/{action}/{tname} dropped rays untouched")
+        kept_out = (full["vignetted"] == 0) & (full["flux"] >= minFlux) & keep
+        assert int(counts.sum()) == int(kept_out.sum())
+
+
 @pytest.mark.parametrize("n", [0, 1, 255, 256, 257, 100_003, 2_000_000])
 def test_select_matches_numpy(n):
     rng = np.random.default_rng(n)
