@@ -10,6 +10,7 @@ import torch
 
 import batoid_amd as bt
 import oracle
+from batoid_amd import _lib
 from oracle import split as OS
 from oracle.host import HostRays, host_records, nthreads
 from helpers import (surfaces, media, random_rays, transforms, assert_cols_identical,
@@ -155,7 +156,7 @@ def test_rsplit_live_skips_dropped_rays(name, surf):
                                   what=f"{name}/{tname} filtered outputs")
 
 
-@pytest.mark.parametrize("action", ["intersect", "reflect"])
+@pytest.mark.parametrize("action", ["intersect", "reflect", "refract"])
 @pytest.mark.parametrize("name,surf", [surfaces()[i] for i in (2, 9, 15)],
                          ids=[surfaces()[i][0] for i in (2, 9, 15)])
 def test_interact_live_skips_dropped_rays(name, surf, action):
@@ -167,7 +168,19 @@ def test_interact_live_skips_dropped_rays(name, surf, action):
     obsc = bt.ObscNegation(bt.ObscAnnulus(0.2, 0.8))
     rng = np.random.default_rng(31)
     minFlux = 1e-3
-    fn = getattr(T, action)
+    med = media()[2][1]
+    if action == "refract":
+        def fn(surf, rv, cs, coating, obscuration, _live_min_flux=None, _live_counts=None):
+            if _live_min_flux is None:
+                return T.refract(surf, rv, bt.vacuum, med, cs, coating=coating,
+                                 obscuration=obscuration)
+            ct = bt.CoordTransform(rv.coordSys, cs)
+            T._interact_live(_lib.ACT_REFRACT, surf, rv, ct, med, coating, obscuration,
+                             _live_min_flux, _live_counts)
+            rv.coordSys = cs
+            return rv
+    else:
+        fn = getattr(T, action)
     for tname, (dr, drot) in transforms():
         cs = _frame(dr, drot)
         base = HostRays(**random_rays(N, seed=33, upward=0.3, spread=0.3))
