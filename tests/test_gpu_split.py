@@ -156,6 +156,29 @@ def test_rsplit_live_skips_dropped_rays(name, surf):
                                   what=f"{name}/{tname} filtered outputs")
 
 
+def test_rsplit_live_analytic_flag_on_other_kinds_fails_rays():
+    """bt_rsplit_live with kinds = BT_PLAN_ANALYTIC on a Sum: the
+    analytic-kinds kernel cannot evaluate it, so every kept ray fails (and
+    is vignetted) in both parts instead of being traced wrongly."""
+    from batoid_amd import _device
+    surf = dict(surfaces())["sum"]
+    assert not surf._analytic
+    med = media()[2][1]
+    coat = bt.SimpleCoating(0.0371, 0.9588)
+    base = HostRays(**random_rays(N, seed=35, upward=0.3, spread=0.3))
+    rv = _gpu(base)
+    split = bt.RayVector._empty(len(rv), rv.coordSys)
+    ct = bt.CoordTransform(rv.coordSys, rv.coordSys)
+    _lib.call("bt_rsplit_live", surf._dev.ptr(), _lib.dvec(ct.dr),
+              _lib.dvec(ct.drot.ravel()), med._dev.ptr(), ctypes.byref(coat._record), None, 0,
+              rv._rays, split._rays, 0.0, 5, _lib.PLAN_ANALYTIC, None, _device.stream())
+    r, sp = _host(rv), _host(split)
+    live = (base["vignetted"] == 0) & (base["failed"] == 0)
+    assert live.any()
+    assert r["failed"][live].all() and r["vignetted"][live].all()
+    assert sp["failed"][live].all() and sp["vignetted"][live].all()
+
+
 @pytest.mark.parametrize("action", ["intersect", "reflect", "refract"])
 @pytest.mark.parametrize("name,surf", [surfaces()[i] for i in (2, 9, 15)],
                          ids=[surfaces()[i][0] for i in (2, 9, 15)])
