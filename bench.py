@@ -732,10 +732,11 @@ def bench_newton_start(nrad, naz, optic_name="perturbed_tma", reps=5, train_step
             + (f", max |dpos| {entry['max_abs_dpos_m']:.2e} m, flags differ "
                f"{entry['flag_mismatches']}" if "flag_mismatches" in entry else ""))
         del work
+    res["_nets"] = nets                         # reused by the batch leg, not reported
     return res
 
 
-def bench_batch(reps=5):
+def bench_batch(reps=5, nets=None):
     """C5: the perturbed TMA, 64 fields (default_rng(1)) x 6 wavelengths =
     384 (field, wavelength) segments with N per segment log-uniform in
     [1e4, 2e5] (default_rng(2)) as an asGrid pupil of nx = round(sqrt(N)),
@@ -820,6 +821,23 @@ def bench_batch(reps=5):
         "flag_mismatches": int((b.rays.torch("failed") != ref_fail).sum()),
     }
     log(f"batch, conic start: {conic_ms:.2f} ms per batch (graph)")
+    if nets is not None:
+        # the learned start of the perturbed TMA's networks (trained on 12
+        # fields at 620 nm by the newton-start leg), over all 384 segments
+        b.set_start(nets)
+        b.capture()
+        mlp_ms = b.time(reps)
+        torch.cuda.synchronize()
+        res["mlp_start"] = {
+            "ms_per_batch_graph": mlp_ms,
+            "rays_per_s": b.n / (mlp_ms * 1e-3),
+            "ray_surfaces_per_s": b.n * nsteps / (mlp_ms * 1e-3),
+            "max_abs_dx_m": float((b.rays.torch("x") - ref_x)[ok].abs().max()),
+            "flag_mismatches": int((b.rays.torch("failed") != ref_fail).sum()),
+        }
+        log(f"batch, learned start: {mlp_ms:.2f} ms per batch (graph), max |dx| "
+            f"{res['mlp_start']['max_abs_dx_m']:.1e} m, flags differ "
+            f"{res['mlp_start']['flag_mismatches']}")
     return res
 
 
@@ -1022,12 +1040,15 @@ def main():
         result["zernike"] = bench_zernike(optic, cpu=not args.no_cpu)
     if not args.no_split:
         result["trace_split"] = bench_trace_split(workloads.tma, cpu=not args.no_cpu)
+    nets = None
     if not args.no_newton_start:
         result["newton_start"] = {
             name: bench_newton_start(args.nrad, args.naz, name)
             for name in ("perturbed_tma", "freeform_tma")}
+        nets = result["newton_start"]["perturbed_tma"].pop("_nets")
+        result["newton_start"]["freeform_tma"].pop("_nets")
     if not args.no_batch:
-        result["batch"] = bench_batch()
+        result["batch"] = bench_batch(nets=nets)
     if not args.no_cpu:
         log("timing the CPU baseline ...")
         result["cpu_baseline"] = cpu_trace(optic, batches[0], "C2 TMA, first batch")
