@@ -290,3 +290,91 @@ def test_nested_sum_records():
     # plan flags: nested Sums need the nested-capable kernel
     from batoid_amd.plan import _has_nested_sum
     assert _has_nested_sum(recs) and not _has_nested_sum(bt.Sum(a, b)._records()[0])
+
+
+def _horner2d_full(x, y, flat, nx, ny):
+    """polynomialSurface.cpp:50-66 in IEEE doubles: rows of nx coefficients,
+    the inner sum over y, the outer over x, each started from 0.0."""
+    x, y = np.float64(x), np.float64(y)
+    r = np.float64(0.0)
+    for i in range(ny - 1, -1, -1):
+        h = np.float64(0.0)
+        for j in range(nx - 1, -1, -1):
+            h = h * y
+            h = h + flat[i * nx + j]
+        r = r * x
+        r = r + h
+    return r
+
+
+def _horner2d_trimmed(x, y, flat, nx, trim):
+    """The device's horner2d_trim: the trim table's last row and row
+    lengths bound the same loops."""
+    x, y = np.float64(x), np.float64(y)
+    r = np.float64(0.0)
+    for i in range(int(trim[0]), -1, -1):
+        h = np.float64(0.0)
+        for j in range(int(trim[1 + i]) - 1, -1, -1):
+            h = h * y
+            h = h + flat[i * nx + j]
+        r = r * x
+        r = r + h
+    return r
+
+
+def test_horner_trim_is_bit_exact_on_finite_arguments():
+    """The Zernike record's trim table (surface._horner_trim): for finite
+    arguments the trimmed loops give the full loops' bits, signed zeros
+    included, on tables with +0.0 runs, -0.0 entries, zero rows (inner and
+    leading) and an all-zero table; a planted wrong length fails."""
+    from batoid_amd.surface import _horner_trim
+    rng = np.random.default_rng(91)
+    tables = []
+    z = bt.Zernike(rng.normal(size=29) * 1e-4, R_outer=1.3)
+    tables.append(z._xycoef.ravel().copy())                 # triangular
+    t = rng.normal(size=(7, 7))
+    t[2, :] = 0.0
+    t[6, :] = 0.0
+    t[4, 3:] = 0.0
+    t[4, 5] = -0.0
+    t[1, 6] = -0.0
+    t[0, :] = -0.0
+    tables.append(t.ravel())
+    tables.append(np.zeros(49))
+    t = rng.normal(size=(7, 7)) * (rng.uniform(size=(7, 7)) < 0.4)
+    tables.append(t.ravel())
+    pts = list(zip(rng.uniform(-2, 2, 60), rng.uniform(-2, 2, 60)))
+    pts += [(0.0, 0.0), (-0.0, 0.0), (0.0, -0.0), (-0.0, -0.0), (1e-300, -1e300),
+            (-3.0, 0.0), (0.0, 2.5), (5e-324, -5e-324)]
+    for flat in tables:
+        trim = _horner_trim([(flat, 7, 7)])
+        assert trim.dtype == np.int32 and len(trim) == 8
+        for x, y in pts:
+            with np.errstate(all="ignore"):
+                a = _horner2d_full(x, y, flat, 7, 7)
+                b = _horner2d_trimmed(x, y, flat, 7, trim)
+            assert np.float64(a).view(np.uint64) == np.float64(b).view(np.uint64) or \
+                (np.isnan(a) and np.isnan(b)), (x, y, a, b)
+    # a length one short of the last nonzero coefficient changes results
+    flat = tables[0]
+    trim = _horner_trim([(flat, 7, 7)])
+    bad = trim.copy()
+    bad[1] -= 1
+    with np.errstate(all="ignore"):
+        assert any(_horner2d_full(x, y, flat, 7, 7) != _horner2d_trimmed(x, y, flat, 7, bad)
+                   for x, y in pts)
+
+
+def test_horner_trim_layout():
+    """Trim table of (coefficients, x-gradient, y-gradient): 1 + ny entries
+    per table, -1 for a table without a coefficient other than +0.0, and the
+    1 x 1 case (gradient tables of 0 rows)."""
+    from batoid_amd.surface import _horner_trim
+    c = np.zeros((3, 3))
+    c[0, 0] = 1.0
+    c[1, 0] = 2.0
+    trim = _horner_trim([(c, 3, 3), (np.zeros((2, 2)), 2, 2), (np.zeros((2, 2)), 2, 2)])
+    assert list(trim) == [1, 1, 1, 0, -1, 0, 0, -1, 0, 0]
+    one = _horner_trim([(np.array([[3.0]]), 1, 1), (np.zeros((1, 1)), 0, 0),
+                        (np.zeros((1, 1)), 0, 0)])
+    assert list(one) == [0, 1, -1, -1]
