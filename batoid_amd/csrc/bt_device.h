@@ -1075,19 +1075,54 @@ BT_DEV bool newton_fast(const bt_surface* s, double x, double y, double z,
     return true;
 }
 
-// Intersection with a fast start: Newton from the base conic's root (the
-// ray origin where the surface has no conic base) plus `correction` (the
-// network's prediction for BT_START_MLP, 0 for BT_START_CONIC).  The
-// reference path decides every ray the fast start does not settle.
+// One Newton update of `dt` on an Asphere alone (its sag and normal at the
+// ray's point at `dt`); `dt` is kept where the update is not finite.
+BT_DEV void asphere_update(const bt_surface* c, double x, double y, double z, double vx,
+                           double vy, double vz, double& dt)
+{
+    const double rPx = x+vx*dt, rPy = y+vy*dt;
+    double sz, nx, ny, nz;
+    AsphereEval::sag_normal(c, rPx, rPy, sz, nx, ny, nz);
+    const double d = (rPx-x)*nx + (rPy-y)*ny + (sz-z)*nz;
+    const double t = d / (nx*vx + ny*vy + nz*vz);
+    if (t == t) dt = t;
+}
+
+// Intersection with a fast start, then Newton on the full surface to the
+// fast tolerance.  Starts:
+// * BT_START_CONIC on a Sum whose first child is an Asphere: that Asphere's
+//   conic root moved by one Newton update on the Asphere (the Sum's other
+//   terms then hold the whole remaining gap, so the Sum's own, costlier,
+//   iteration takes one update where the conic root needs up to two);
+// * BT_START_CONIC on another Sum: the reference's own start, its first
+//   child's root (sum.cpp:53), so the fast iteration runs the reference's
+//   iterates;
+// * otherwise the base conic's root (the ray origin where the surface has
+//   no conic base) plus `correction` (the network's prediction for
+//   BT_START_MLP, 0 for BT_START_CONIC).
+// The reference path decides every ray the fast start does not settle.
 template <int K = kKindsAll>
 BT_DEV bool fast_tti(const bt_surface* s, double x, double y, double z, double vx,
                      double vy, double vz, double& dt, int niter, double correction,
                      NormalMemo* memo)
 {
     if (newton_surface(s)) {
-        double d0;
-        const bool ok = conic_start<K>(s, x, y, z, vx, vy, vz, d0);
-        d0 += correction;
+        double d0 = 0.0;
+        bool ok;
+        if (K != kKindsAnalytic && correction == 0.0 && s->kind == BT_SUM) {
+            const bt_surface* c = s + 1;
+            if (c->kind == BT_ASPHERE) {
+                ok = quadric_tti(c, x, y, z, vx, vy, vz, d0);
+                if (ok) asphere_update(c, x, y, z, vx, vy, vz, d0);
+            } else if constexpr (K != kKindsSumAPB) {
+                ok = surf_tti<K>(c, x, y, z, vx, vy, vz, d0, niter);
+            } else {
+                ok = false;                     // not a Sum(Asphere, Polynomial, Bicubic)
+            }
+        } else {
+            ok = conic_start<K>(s, x, y, z, vx, vy, vz, d0);
+            d0 += correction;
+        }
         bool conv = false;
         if (ok && d0 == d0) {
             if (s->kind == BT_ASPHERE)

@@ -503,8 +503,10 @@ class CompoundOptic(Optic):
         the requested order.
 
         ``start`` picks how Newton surfaces seed their intersection:
-        "reference" (default; bit-identical to the reference), "conic" (the
-        base conic's root, converged to a 1e-14 m sag gap), or a mapping
+        "reference" (default; bit-identical to the reference), "conic" (an
+        analytic start -- the base conic's root, one Newton update on the
+        leading Asphere of a Sum, or a Sum's first child's reference root --
+        converged to a 1e-14 m sag gap), or a mapping
         {interface name: NewtonStartNet} (learned starts, see
         newton_start.py).  Fast starts fall back to the reference path for
         any ray they do not converge."""

@@ -163,22 +163,28 @@ enum bt_action { BT_ACT_INTERSECT = 0, BT_ACT_REFLECT = 1, BT_ACT_REFRACT = 2 };
  *   REFERENCE  the reference's start and exactly niter iterations
  *              (surface.cpp:22-49, asphere.cpp:75-85, sum.cpp:47-56):
  *              bit-identical to the reference;
- *   CONIC      the analytic root of the base conic (Asphere's quadric, or a
- *              Sum's first child), then Newton on the full surface until the
- *              sag gap is <= 1e-14 m (at most niter iterations);
- *   MLP        as CONIC, plus the learned correction of the step's network.
+ *   CONIC      an analytic start, then Newton on the full surface until the
+ *              sag gap is <= 1e-14 m (at most niter iterations).  The start:
+ *              an Asphere's quadric root; for a Sum led by an Asphere, that
+ *              Asphere's quadric root moved by one Newton update on the
+ *              Asphere; for any other Sum, the reference's own start (its
+ *              first child's root, sum.cpp:53); the ray origin for a
+ *              polynomial or Bicubic surface;
+ *   MLP        the base conic's root (a Sum's first leaf's; the ray origin
+ *              without one) plus the learned correction of the step's
+ *              network, then the same iteration.
  * A ray that does not converge from a fast start within niter iterations is
  * re-intersected on the REFERENCE path, which decides its flags.  Where the
- * fast start is the reference's own start (CONIC on an Asphere: the same
- * iterates), a ray the reference fails is therefore failed here too.  From
- * a different start (a Sum's first-child root, an MLP prediction) a fast
- * start can converge on a ray whose reference iteration would not have
- * converged in niter steps: that ray is intersected where the reference
- * marks it failed.  Positions agree with the reference to <= 1e-12 m where
- * both intersect.  Starts other
- * than REFERENCE are honoured only by bt_trace_plan with BT_PLAN_FAST_START
- * set; MLP starts only with BT_PLAN_MLP_START also set, and an MLP step with
- * a null network runs as CONIC. */
+ * fast start is the reference's own start (CONIC on an Asphere, and on a
+ * Sum not led by an Asphere: the same iterates), a ray the reference fails
+ * is therefore failed here too.  From a different start (CONIC on a Sum led
+ * by an Asphere, an MLP prediction) a fast start can converge on a ray
+ * whose reference iteration would not have converged in niter steps: that
+ * ray is intersected where the reference marks it failed.  Positions agree
+ * with the reference to <= 1e-12 m where both intersect.  Starts other than
+ * REFERENCE are honoured only by bt_trace_plan with BT_PLAN_FAST_START set;
+ * MLP starts only with BT_PLAN_MLP_START also set, and an MLP step with a
+ * null network runs as CONIC. */
 enum bt_start { BT_START_REFERENCE = 0, BT_START_CONIC = 1, BT_START_MLP = 2 };
 
 typedef struct bt_step {

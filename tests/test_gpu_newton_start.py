@@ -255,3 +255,30 @@ def test_conic_start_keeps_the_reference_iteration_budget():
     for k in ("x", "y", "z", "t"):
         np.testing.assert_allclose(getattr(fast, k)[ok], getattr(ref, k)[ok], rtol=0,
                                    atol=1e-12)
+
+
+def test_conic_start_on_a_sum_runs_the_reference_iterates():
+    """A CONIC start on a Sum not led by an Asphere is the reference's own
+    start (the first child's root), so the fast iteration runs the
+    reference's iterates and stops within its niter = 5: on rays that need
+    more (test_oracle_golden.slow_sum_case: thousands fail at 5, not at 8)
+    the flags are the reference's."""
+    from test_oracle_golden import slow_sum_case
+    surf, rays = slow_sum_case()
+    optic = bt.CompoundOptic([bt.Interface(surf, name="S")], name="slow_sum",
+                             inMedium=bt.vacuum)
+    base = bt.RayVector(**{k: rays[k] for k in ("x", "y", "z", "vx", "vy", "vz", "t",
+                                                 "wavelength", "flux")},
+                        vignetted=rays["vignetted"], failed=rays["failed"])
+    ref = base.copy()
+    optic.trace(ref)
+    fast = base.copy()
+    optic.trace(fast, start="conic")
+    torch.cuda.synchronize()
+    assert ref.failed.sum() > 5000
+    np.testing.assert_array_equal(fast.failed, ref.failed)
+    np.testing.assert_array_equal(fast.vignetted, ref.vignetted)
+    ok = ~ref.failed
+    for k in ("x", "y", "z", "t"):
+        np.testing.assert_allclose(getattr(fast, k)[ok], getattr(ref, k)[ok], rtol=0,
+                                   atol=1e-12)

@@ -309,3 +309,32 @@ def test_slow_newton_case_needs_more_than_niter():
         failed[niter] = h["failed"].astype(bool)
     assert failed[5].sum() > 500
     assert not failed[6].any() and not failed[8].any()
+
+
+def slow_sum_case():
+    """A Sum led by a weak Zernike with the strong asphere above as its
+    second term: the reference starts the Sum's Newton at the Zernike's
+    root (sum.cpp:50-55), nearly the plane z = 0, and its iteration from
+    there needs more than niter = 5 steps on thousands of the oblique
+    rays."""
+    surf = bt.Sum([bt.Zernike([0, 0, 0, 0, 1e-4], R_outer=1.5),
+                   bt.Asphere(3.0, -0.5, [0.05, 0.01])])
+    rays = random_rays(20000, seed=1, z0=2.0, spread=0.8, radius=1.2, prefail=0.0,
+                       previg=0.0)
+    return surf, rays
+
+
+def test_slow_sum_case_needs_more_than_niter():
+    """The reference fails thousands of these rays at niter = 5, thousands
+    fewer at 8 (the rest miss at any budget)."""
+    surf, rays = slow_sum_case()
+    tab = host_records(surf)
+    failed = {}
+    for niter in (5, 8):
+        h = HostRays(**rays)
+        oracle.load().or_intersect(tab.ptr(), _dv(np.zeros(3)), _dv(np.eye(3)), h.struct,
+                                   niter, 1)
+        failed[niter] = h["failed"].astype(bool)
+    assert failed[5].sum() > 5000
+    assert failed[5].sum() - failed[8].sum() > 5000
+    assert not (failed[8] & ~failed[5]).any()
