@@ -146,7 +146,8 @@ def rSplit(surface, rv, inMedium, outMedium, coating, coordSys=None, obscuration
 
     ``_live_min_flux`` (traceSplit's own use): ``rv`` still holds rays that
     traceSplit's filter dropped (vignetted, or flux below this threshold);
-    they are skipped and only marked dropped in the reflected part.
+    they are skipped and only marked dropped in the reflected part, and a
+    reflected ray that filter drops is written only as its flux and flags.
     ``_live_counts``: a device int64[2] the numbers of refracted and
     reflected rays passing that filter are added to."""
     from .rayVector import RayVector

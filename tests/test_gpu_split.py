@@ -111,7 +111,9 @@ def test_rsplit_live_skips_dropped_rays(name, surf):
     exactly as they were and their split part is marked vignetted; every
     kept ray is split bit for bit as bt_rsplit splits it (the analytic-kinds
     kernel for the sphere and the asphere, the all-kinds one for the Sum),
-    and the kernel's counts of kept output rays are the filter's."""
+    except that a reflected ray the filter drops is written only as its
+    flux and flags; and the kernel's counts of kept output rays are the
+    filter's."""
     med = media()[2][1]
     coat = bt.SimpleCoating(0.0371, 0.9588)
     obsc = bt.ObscNegation(bt.ObscAnnulus(0.2, 0.8))
@@ -120,7 +122,8 @@ def test_rsplit_live_skips_dropped_rays(name, surf):
     for tname, (dr, drot) in transforms():
         cs = _frame(dr, drot)
         base = HostRays(**random_rays(N, seed=27, upward=0.3, spread=0.3))
-        flux = rng.uniform(0.0, 3e-3, N)
+        # reflected rays (x 0.0371) pass minFlux from flux 0.027 up
+        flux = rng.uniform(0.0, 3e-2, N)
         flux[rng.uniform(size=N) < 0.02] = np.nan
         base.cols["flux"] = flux
         base.cols["vignetted"] = (rng.uniform(size=N) < 0.2).astype(np.uint8)
@@ -136,13 +139,19 @@ def test_rsplit_live_skips_dropped_rays(name, surf):
         # the kernel's own counts of the rays each output keeps
         for c, h in zip(counts.view(2, 64).sum(1).tolist(), (fr, fs)):
             assert c == int(((h["vignetted"] == 0) & (h["flux"] >= minFlux) & keep).sum())
+        skept = keep & (fs["vignetted"] == 0) & (fs["flux"] >= minFlux)
+        sdrop = keep & ~skept
+        assert skept.sum() > 0 and sdrop.sum() > 0
         for k in FCOLS + BCOLS:
             assert_cols_identical({k: lr[k][keep]}, {k: fr[k][keep]}, cols=(k,),
                                   what=f"{name}/{tname} kept refracted")
-            assert_cols_identical({k: ls[k][keep]}, {k: fs[k][keep]}, cols=(k,),
+            assert_cols_identical({k: ls[k][skept]}, {k: fs[k][skept]}, cols=(k,),
                                   what=f"{name}/{tname} kept reflected")
             assert_cols_identical({k: lr[k][~keep]}, {k: base[k][~keep]}, cols=(k,),
                                   what=f"{name}/{tname} dropped rays untouched")
+        for k in ("flux",) + BCOLS:
+            assert_cols_identical({k: ls[k][sdrop]}, {k: fs[k][sdrop]}, cols=(k,),
+                                  what=f"{name}/{tname} reflected rays the filter drops")
         assert ls["vignetted"][~keep].all()
         # the filter after the split keeps the same rays either way
         for a, b in ((live_r, full_r), (live_s, full_s)):
@@ -207,7 +216,8 @@ def test_interact_live_skips_dropped_rays(name, surf, action):
     for tname, (dr, drot) in transforms():
         cs = _frame(dr, drot)
         base = HostRays(**random_rays(N, seed=33, upward=0.3, spread=0.3))
-        flux = rng.uniform(0.0, 3e-3, N)
+        # reflected rays (x 0.0371) pass minFlux from flux 0.027 up
+        flux = rng.uniform(0.0, 3e-2, N)
         flux[rng.uniform(size=N) < 0.02] = np.nan
         base.cols["flux"] = flux
         base.cols["vignetted"] = (rng.uniform(size=N) < 0.2).astype(np.uint8)
