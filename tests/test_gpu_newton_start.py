@@ -282,3 +282,34 @@ def test_conic_start_on_a_sum_runs_the_reference_iterates():
     for k in ("x", "y", "z", "t"):
         np.testing.assert_allclose(getattr(fast, k)[ok], getattr(ref, k)[ok], rtol=0,
                                    atol=1e-12)
+
+
+@pytest.mark.parametrize("name", ["sum", "sum_plane", "sum_nested", "sum_deep", "sum_wide"])
+def test_conic_start_on_every_sum_layout(name):
+    """CONIC starts on Sums traced by each plan kernel: Sum(Asphere,
+    Zernike, Bicubic) (its Asphere's updated quadric root), a Sum led by a
+    Plane, Sums of Sums to three levels and an eleven-child Sum led by a
+    Quadric (the first child's reference root).  Flags identical to the
+    reference start, positions and times within 1e-12 m, directions within
+    1e-14."""
+    from helpers import surfaces, random_rays
+    surf = dict(surfaces())[name]
+    print(f"case conic start {name}", flush=True)
+    optic = bt.CompoundOptic([bt.Interface(surf, name="S")], name=name, inMedium=bt.vacuum)
+    rays = random_rays(20000, seed=71, z0=1.0, spread=0.15)
+    base = bt.RayVector(**{k: rays[k] for k in ("x", "y", "z", "vx", "vy", "vz", "t",
+                                                 "wavelength", "flux")},
+                        vignetted=rays["vignetted"], failed=rays["failed"])
+    ref = base.copy()
+    optic.trace(ref)
+    fast = base.copy()
+    optic.trace(fast, start="conic")
+    a, b = _cols(ref), _cols(fast)
+    assert np.array_equal(a["failed"], b["failed"])
+    assert np.array_equal(a["vignetted"], b["vignetted"])
+    ok = ~a["failed"]
+    assert ok.mean() > 0.9
+    for c in ("x", "y", "z", "t"):
+        assert np.abs(a[c][ok] - b[c][ok]).max() <= 1e-12, (name, c)
+    for c in ("vx", "vy", "vz"):
+        assert np.abs(a[c][ok] - b[c][ok]).max() <= 1e-14, (name, c)
