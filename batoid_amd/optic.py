@@ -178,9 +178,16 @@ class Interface(Optic):
         if live and _counts is not None:
             base, off = _counts
             live_counts = base[off * _COUNT_SLOTS:(off + 2) * _COUNT_SLOTS]
+        bound = getattr(rv, '_flux_bound', None)
         refracted, reflected = self.rSplit(
             rv, reverse=reverse, obscuration=self.obscuration,
             _live_min_flux=minFlux if _filtered else None, _live_counts=live_counts)
+        if bound is not None:
+            ft, fr = self._split_fractions(reverse)
+            if refracted is not None:
+                refracted._flux_bound = bound * abs(ft)
+            if reflected is not None:
+                reflected._flux_bound = bound * abs(fr)
         path = list(getattr(rv, 'path', [])) + [self.name]
         for k, r in enumerate((refracted, reflected)):
             if r is not None:
@@ -194,17 +201,23 @@ class Interface(Optic):
             rForward, rReverse = rReverse, rForward
         return rForward, rReverse
 
-    def _split_batch(self, inputs, minFlux, _verbose=False, _compact=True):
+    def _split_batch(self, inputs, minFlux, _verbose=False, _compact=True, _deferred=False):
         """traceSplit of several (rays, reverse, filtered) inputs (no host
         synchronisation: the launches queue on the stream); the filtered
-        inputs' splits count their kept rays into one device array."""
+        inputs' splits count their kept rays into one device array (not
+        when ``_deferred``: nobody reads them)."""
         counts = None
-        if any(filt for _, _, filt in inputs):
+        if not _deferred and any(filt for _, _, filt in inputs):
             counts = torch.zeros(2 * _COUNT_SLOTS * len(inputs), dtype=torch.int64,
                                  device=_device.device())
         return [self.traceSplit(rv, minFlux, rev, _verbose, filt,
                                 (counts, 2 * j) if counts is not None else None)
                 for j, (rv, rev, filt) in enumerate(inputs)]
+
+    def _split_fractions(self, reverse):
+        """Factors rSplit applies to the flux of its (refracted, reflected)
+        parts: 1 for a step without a coating (Mirror, Baffle)."""
+        return 1.0, 1.0
 
     @cached_property
     def R_outer(self):
@@ -283,6 +296,10 @@ class RefractiveInterface(Interface):
                                    obscuration=obscuration, _live_min_flux=_live_min_flux,
                                    _live_counts=_live_counts)
 
+    def _split_fractions(self, reverse):
+        coating = self.forwardCoating if reverse else self.reverseCoating
+        return coating.transmissivity, coating.reflectivity
+
     def _interact(self, rv, reverse, obscuration):
         from .trace import refract
         m1, m2 = (self.outMedium, self.inMedium) if reverse else (self.inMedium, self.outMedium)
@@ -329,6 +346,9 @@ class Detector(Interface):
         return self.surface.rSplit(rv, self.inMedium, self.outMedium, self.forwardCoating,
                                    coordSys=self.coordSys, obscuration=obscuration,
                                    _live_min_flux=_live_min_flux, _live_counts=_live_counts)
+
+    def _split_fractions(self, reverse):
+        return self.forwardCoating.transmissivity, self.forwardCoating.reflectivity
 
 
 class Baffle(Interface):
@@ -595,9 +615,63 @@ class CompoundOptic(Optic):
             s = "reverse" if reverse else "forward"
             print("traceSplit {}       {:30s} flux = {:18.8f}   nphot = {:10d}".format(
                 s, self.name, *_live_stats(rv, None)))
-        return self._split_batch([(rv, reverse, False)], minFlux, _verbose)[0]
+        bound = None if _verbose else self._flux_bound(rv, minFlux)
+        if bound is None:
+            return self._split_batch([(rv, reverse, False)], minFlux, _verbose)[0]
+        rv._flux_bound = bound
+        try:
+            return self._split_batch([(rv, reverse, False)], minFlux, _deferred=True)[0]
+        finally:
+            rv.__dict__.pop('_flux_bound', None)
 
-    def _split_batch(self, inputs, minFlux, _verbose=False, _compact=True):
+    def _interfaces(self):
+        for item in self.items:
+            if isinstance(item, CompoundOptic):
+                yield from item._interfaces()
+            else:
+                yield item
+
+    def _flux_bound(self, rv, minFlux):
+        """The largest |flux| among the unvignetted rays of ``rv``, when
+        traceSplit may run without reading back counts (one read-back here;
+        None otherwise).
+
+        Every coating is constant, so a ray set's fluxes are bounded by this
+        bound times the |transmissivity| and |reflectivity| factors along
+        its path (rounded products are monotonic, so the bound computed in
+        the same order holds bit for bit; vignetted rays stay vignetted); a
+        set whose bound falls below minFlux holds no ray the filter keeps
+        and is dropped unread.  That ends the recursion when minFlux > 0,
+        every reflectivity that turns rays back (RefractiveInterface,
+        Detector) is below 1 in magnitude, the bound is finite, and no
+        unvignetted ray at or above minFlux is already failed (a failed
+        ray's flux is not scaled)."""
+        if not minFlux > 0:
+            return None
+        for item in self._interfaces():
+            if item.skip or isinstance(item, (Mirror, Baffle)):
+                continue
+            if not isinstance(item, (RefractiveInterface, Detector)):
+                return None
+            for reverse in ((False,) if isinstance(item, Detector) else (False, True)):
+                try:
+                    ft, fr = (abs(float(f)) for f in item._split_fractions(reverse))
+                except (AttributeError, TypeError, ValueError):
+                    return None
+                if not (fr < 1 and ft < float('inf')):
+                    return None
+        if len(rv) == 0:
+            return 0.0
+        flux = rv.torch("flux")
+        unvig = rv.torch("vignetted") == 0
+        top = torch.where(unvig & ~torch.isnan(flux), flux.abs(), torch.zeros_like(flux)).max()
+        bad = (unvig & (flux >= minFlux) & (rv.torch("failed") != 0)).any()
+        top, bad = torch.stack([top, bad.to(flux.dtype)]).tolist()
+        if bad or not top < float('inf'):
+            return None
+        return top
+
+    def _split_batch(self, inputs, minFlux, _verbose=False, _compact=True, _deferred=False):
         """traceSplit of several (rays, reverse, filtered) inputs at once.
 
         The reference pops one work item at a time from a LIFO stack and
@@ -618,11 +692,20 @@ class CompoundOptic(Optic):
         optic are copied out, kept rays in order (``_compact``; a nested
         optic returns its sets uncopied, with their counts), so each
         returned set holds exactly the rays the reference's per-item copies
-        would."""
+        would.
+
+        ``_deferred`` (the inputs carry ``_flux_bound``, see _flux_bound): no
+        counts are read back between rounds; a set is dropped when its flux
+        bound is below minFlux and otherwise traced on (if all its rays were
+        vignetted, as a pass that skips every ray), and the sets leaving the
+        outermost optic are counted once, at the end, the empty ones
+        dropped."""
         if self.skip:
             for rv, _, _ in inputs:
                 rv._live_slot = rv._live_count = None
             return [((None, rv) if rev else (rv, None)) for rv, rev, _ in inputs]
+        if _deferred:
+            return self._split_deferred(inputs, minFlux, _compact)
         last = len(self.items) - 1
         # work entries: (input, key, rays, reverse, item index, filtered)
         work = [(k, (), rv, rev, last if rev else 0, filt)
@@ -666,6 +749,60 @@ class CompoundOptic(Optic):
                     pushed[parent] = p + 1
                     work.append((k, key + (-p,), rr, side == 1,
                                  index + 1 if side == 0 else index - 1, True))
+        return [tuple([r for _, r in sorted(lst, key=lambda e: e[0])] for lst in o)
+                for o in outs]
+
+    def _split_deferred(self, inputs, minFlux, compact):
+        """_split_batch without read-backs between rounds: the same rounds,
+        keys and order, with a set dropped by its flux bound instead of its
+        count (a set the bound keeps but the filter empties goes on empty,
+        and leaves no output)."""
+        last = len(self.items) - 1
+        work = [(k, (), rv, rev, last if rev else 0, filt)
+                for k, (rv, rev, filt) in enumerate(inputs)]
+        outs = [([], []) for _ in inputs]
+        leaving = []                                   # (input, side, order, rays)
+        while work:
+            groups = {}
+            for w in work:
+                groups.setdefault(w[4], []).append(w)
+            work = []
+            pushed, seq = {}, {}
+            for index in sorted(groups):
+                entries = groups[index]
+                results = self.items[index]._split_batch(
+                    [(w[2], w[3], w[5]) for w in entries], minFlux, _compact=False,
+                    _deferred=True)
+                for w, (rF, rR) in zip(entries, results):
+                    k, key = w[0], w[1]
+                    parent = id(w)
+                    for side, rl in enumerate((rF, rR)):
+                        if not isinstance(rl, list):   # a skipped item passes rays on
+                            rl = [rl]
+                        leaves = (side == 0 and w[4] == last) or (side == 1 and w[4] == 0)
+                        for rr in rl:
+                            if not rr or not rr._flux_bound >= minFlux:
+                                continue
+                            if leaves:
+                                s = seq.get(parent, 0)
+                                seq[parent] = s + 1
+                                leaving.append((k, side, (key, s), rr))
+                            else:
+                                p = pushed.get(parent, 0)
+                                pushed[parent] = p + 1
+                                work.append((k, key + (-p,), rr, side == 1,
+                                             w[4] + 1 if side == 0 else w[4] - 1, True))
+        if compact and leaving:
+            sel = RayVector._count_select([rr for _, _, _, rr in leaving], minFlux)
+            for i, ((k, side, order, rr), m) in enumerate(zip(leaving, sel.counts.tolist())):
+                if m:
+                    out = sel.scatter(i, m)
+                    out.path = rr.path
+                    outs[k][side].append((order, out))
+        else:
+            for k, side, order, rr in leaving:
+                rr._live_slot = rr._live_count = None
+                outs[k][side].append((order, rr))
         return [tuple([r for _, r in sorted(lst, key=lambda e: e[0])] for lst in o)
                 for o in outs]
 

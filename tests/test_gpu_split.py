@@ -372,3 +372,62 @@ def test_traceSplit_golden_optics(name):
     assert len(rF) >= 1 and len(rF) + len(rR) > 2
     _check_paths(optic, rays, rF, rR, 1e-12)
     _check_oracle(optic, rays, rF, rR, minFlux)
+
+
+def _coated_lsst():
+    optic = bt.Optic.fromYaml(os.path.join(GOLD, "optics", "LSST_r.yaml"))
+    for item in optic.itemDict.values():
+        if isinstance(item, bt.RefractiveInterface):
+            item.forwardCoating = bt.SimpleCoating(0.02, 0.98)
+            item.reverseCoating = bt.SimpleCoating(0.02, 0.98)
+        if isinstance(item, bt.Detector):
+            item.forwardCoating = bt.SimpleCoating(0.02, 0.98)
+    return optic
+
+
+def test_traceSplit_deferred_equals_counted_rounds(monkeypatch):
+    """traceSplit without count read-backs between rounds (sets dropped by
+    their flux bound, the leaving sets counted once at the end) and with a
+    read-back per round give the same sets in the same order, bit for
+    bit."""
+    optic = _coated_lsst()
+    rays = bt.RayVector.asPolar(optic, wavelength=620e-9, theta_x=np.deg2rad(0.3),
+                                theta_y=np.deg2rad(-0.2), nrad=30, naz=180)
+    for minFlux in (1e-3, 1e-4):
+        assert optic._flux_bound(rays, minFlux) == 1.0
+        dF, dR = optic.traceSplit(rays.copy(), minFlux=minFlux)
+        with monkeypatch.context() as mp:
+            mp.setattr(bt.CompoundOptic, "_flux_bound", lambda self, rv, m: None)
+            cF, cR = optic.traceSplit(rays.copy(), minFlux=minFlux)
+        for got, want in ((dF, cF), (dR, cR)):
+            assert [g.path for g in got] == [w.path for w in want]
+            for g, w in zip(got, want):
+                assert_cols_identical(_host(g), _host(w), what="/".join(g.path))
+        assert len(dF) + len(dR) > 4
+
+
+def test_traceSplit_flux_bound_preconditions():
+    """The deferred traceSplit runs only where its recursion provably ends:
+    minFlux > 0, every reflectivity that turns rays back below 1, a finite
+    largest flux, and no failed ray the filter would keep."""
+    optic = _coated_lsst()
+    rays = bt.RayVector.asPolar(optic, wavelength=620e-9, theta_x=0.0, theta_y=0.0,
+                                nrad=5, naz=30)
+    i, j = (int(q) for q in np.flatnonzero(~np.asarray(rays.vignetted, bool))[[3, 5]])
+    assert optic._flux_bound(rays, 1e-3) == 1.0
+    assert optic._flux_bound(rays, 0.0) is None
+    r = rays.copy()
+    r.torch("flux")[i] = float("inf")
+    assert optic._flux_bound(r, 1e-3) is None
+    r = rays.copy()
+    r.torch("failed")[i] = 1
+    assert optic._flux_bound(r, 1e-3) is None
+    r.torch("vignetted")[i] = 1                     # failed and vignetted: dropped anyway
+    assert optic._flux_bound(r, 1e-3) == 1.0
+    r = rays.copy()
+    r.torch("flux")[i] = 7.0
+    r.torch("flux")[j] = float("nan")
+    assert optic._flux_bound(r, 1e-3) == 7.0
+    item = next(i for i in optic.itemDict.values() if isinstance(i, bt.RefractiveInterface))
+    item.reverseCoating = bt.SimpleCoating(1.0, 0.0)
+    assert optic._flux_bound(rays, 1e-3) is None
