@@ -393,9 +393,25 @@ def test_traceSplit_deferred_equals_counted_rounds(monkeypatch):
     optic = _coated_lsst()
     rays = bt.RayVector.asPolar(optic, wavelength=620e-9, theta_x=np.deg2rad(0.3),
                                 theta_y=np.deg2rad(-0.2), nrad=30, naz=180)
+    counted = []
+    real = bt.RayVector._count_select
+
+    def spy(rvs, minFlux):
+        sel = real(rvs, minFlux)
+        counted.append(sel.counts.tolist())
+        return sel
+
+    empties = 0
     for minFlux in (1e-3, 1e-4):
         assert optic._flux_bound(rays, minFlux) == 1.0
-        dF, dR = optic.traceSplit(rays.copy(), minFlux=minFlux)
+        counted.clear()
+        with monkeypatch.context() as mp:
+            mp.setattr(bt.RayVector, "_count_select", staticmethod(spy))
+            dF, dR = optic.traceSplit(rays.copy(), minFlux=minFlux)
+        # one count pass, at the end; the empty sets leave no output
+        assert len(counted) == 1
+        assert len(dF) + len(dR) == sum(c > 0 for c in counted[0])
+        empties += counted[0].count(0)
         with monkeypatch.context() as mp:
             mp.setattr(bt.CompoundOptic, "_flux_bound", lambda self, rv, m: None)
             cF, cR = optic.traceSplit(rays.copy(), minFlux=minFlux)
@@ -404,6 +420,8 @@ def test_traceSplit_deferred_equals_counted_rounds(monkeypatch):
             for g, w in zip(got, want):
                 assert_cols_identical(_host(g), _host(w), what="/".join(g.path))
         assert len(dF) + len(dR) > 4
+    # sets the flux bound kept but vignetting had emptied were traced on
+    assert empties > 0
 
 
 def test_traceSplit_flux_bound_preconditions():
