@@ -12,20 +12,22 @@
 // which holds the least-squares normal matrix (r, c < J), the projections
 // sum_i Z_j w (c = J) and sum_i w^2.  The host solves the J x J system.
 //
-// Each workgroup stages the coefficient table in LDS and takes chunks of 64
-// rays; both products run on the FP64
-// matrix cores (v_mfma_f64_16x16x4f64: lane l supplies A[l%16][l/16] and
-// B[l/16][l%16] and holds D[4e + l/16][l%16], e = 0..3):
-//  1. monomials of the 64 rays and their wavefront values to LDS;
-//  2. the basis values Z[p][j] = sum_m mono[p][m] C[j][m]: wave w takes rays
-//     16w .. 16w+15 against every 16-column block of C (K = monomials);
+// Each workgroup stages the coefficient table in LDS; each of its waves
+// then takes groups of 16 rays on its own, and both products run on the
+// FP64 matrix cores (v_mfma_f64_16x16x4f64: lane l supplies A[l%16][l/16]
+// and B[l/16][l%16] and holds D[4e + l/16][l%16], e = 0..3):
+//  1. lane l computes the powers of ray l % 16 and, in registers, the
+//     monomials it supplies as A: 4 ks + l / 16 for each K step ks;
+//  2. the basis values Z[p][j] = sum_m mono[p][m] C[j][m] of the 16 rays
+//     against every 16-column block of C (K = monomials), written with the
+//     wavefront values to the wave's own LDS rows;
 //  3. the Gram blocks G[16 rb..][16 cb..] (rb <= cb) += sum_p B[p][r] B[p][c]
 //     over the same 16 rays (K = rays), accumulated in registers.
-// A wave only reads the basis columns it wrote, so steps 2-3 need no
-// barrier.  At the end the four waves' Gram blocks are added in wave order
-// in LDS and the workgroup writes its partial Gram matrix (upper triangle,
+// A wave only reads the LDS rows it wrote, so the ray loop has no barrier.
+// At the end the four waves' Gram blocks are added in wave order in LDS
+// and the workgroup writes its partial Gram matrix (upper triangle,
 // row-major); a second kernel sums the partials in workgroup order, so the
-// result does not depend on scheduling.  Chunks whose rays are all masked
+// result does not depend on scheduling.  Groups whose rays are all masked
 // are skipped (they add exact zeros).
 #include <hip/hip_runtime.h>
 #include <string>
@@ -64,42 +66,98 @@ constexpr int zernike_ksteps(int nb)
     return nb == 1 ? 4 : nb == 2 ? 9 : nb == 3 ? 14 : nb == 4 ? 17 : 20;
 }
 
-// NB = ceil((J + 1) / 16) Gram blocks per side
+// Radial order d and index q of monomial m = d(d+1)/2 + q (x^(d-q) y^q).
+constexpr int mono_d(int m)
+{
+    int d = 0;
+    while ((d + 1) * (d + 2) / 2 <= m) d++;
+    return d;
+}
+constexpr int mono_q(int m) { return m - mono_d(m) * (mono_d(m) + 1) / 2; }
+
+// Monomial m of a ray from its powers (0 past radial order kZMaxN).
+template <int m>
+__device__ __forceinline__ double mono_value(const double (&up)[kZMaxN + 1],
+                                             const double (&vp)[kZMaxN + 1])
+{
+    constexpr int d = mono_d(m), q = mono_q(m);
+    if constexpr (d > kZMaxN) {
+        return 0.0;
+    } else {
+        return up[d - q] * vp[q];
+    }
+}
+
+// The A operand of K step KSI for lane group lk: monomial 4 KSI + lk of the
+// lane's ray, 0 at and past M.
+template <int KSI>
+__device__ __forceinline__ double mono_operand(const double (&up)[kZMaxN + 1],
+                                               const double (&vp)[kZMaxN + 1], int lk, int M)
+{
+    const double c0 = mono_value<4 * KSI + 0>(up, vp);
+    const double c1 = mono_value<4 * KSI + 1>(up, vp);
+    const double c2 = mono_value<4 * KSI + 2>(up, vp);
+    const double c3 = mono_value<4 * KSI + 3>(up, vp);
+    const double v = lk == 0 ? c0 : lk == 1 ? c1 : lk == 2 ? c2 : c3;
+    return 4 * KSI + lk < M ? v : 0.0;
+}
+
+// The basis product of K steps KSI .. KS-1 (unrolled; steps at and past
+// ksteps add nothing and are skipped).
+template <int NB, int KS, int KSI>
+__device__ __forceinline__ void basis_steps(v4d (&z)[NB], const double (&up)[kZMaxN + 1],
+                                            const double (&vp)[kZMaxN + 1], int lk, int li,
+                                            int M, int ksteps, const double (*ct)[NB * 16])
+{
+    if constexpr (KSI < KS) {
+        if (KSI < ksteps) {
+            const double am = mono_operand<KSI>(up, vp, lk, M);
+            const int m = KSI * 4 + lk;
+#pragma unroll
+            for (int jb = 0; jb < NB; jb++)
+                z[jb] = __builtin_amdgcn_mfma_f64_16x16x4f64(am, ct[m][jb * 16 + li], z[jb], 0,
+                                                             0, 0);
+            basis_steps<NB, KS, KSI + 1>(z, up, vp, lk, li, M, ksteps, ct);
+        }
+    }
+}
+
+// NB = ceil((J + 1) / 16) Gram blocks per side.  Each wave works alone on
+// groups of 16 rays (group g: rays 16 g .. 16 g + 15, the groups dealt to
+// the grid's waves in turn): lane l handles ray l % 16 and computes, in
+// registers, exactly the monomials it supplies as the basis product's A
+// operand (4 ks + l / 16); the basis values go through the wave's own LDS
+// rows to the Gram product.  No barrier inside the ray loop.
 template <int NB>
 __global__ __launch_bounds__(kZThreads) void k_zernike_gram(ZArgs a)
 {
     constexpr int NBLK = NB * (NB + 1) / 2;
     constexpr int KS = zernike_ksteps(NB);
     constexpr int MP = KS * 4, JP = NB * 16;         // padded monomial and basis counts
-    __shared__ double mono[MP][kZChunk];    // [monomial][ray]; later the wave sums
-    __shared__ double basis[kZChunk][JP];   // [ray][basis j]; j = J is w
-    __shared__ double ct[MP][JP];           // C transposed, [monomial][basis j]
+    constexpr int kWaves = kZThreads / 64;
+    __shared__ double ct[MP][JP];                    // C transposed, [monomial][basis j]
+    __shared__ double basis[kWaves][16][JP];         // per wave: [ray][basis j]; j = J is w
     const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;
     const int li = lane & 15, lk = lane >> 4;
     const int J = a.J, J1 = J + 1, M = a.M;
     const int ksteps = (M + 3) / 4;
-    // the table, zero past J and M; monomial rows past M and basis columns
-    // past J stay zero for every chunk
     for (int e = tid; e < MP * JP; e += kZThreads) {
         const int m = e / JP, j = e % JP;
         ct[m][j] = (j < J && m < M) ? a.coef[(int64_t)j * M + m] : 0.0;
     }
-    for (int e = tid; e < (MP - M) * kZChunk; e += kZThreads)
-        mono[M + e / kZChunk][e % kZChunk] = 0.0;
-    for (int e = tid; e < kZChunk * (JP - J1); e += kZThreads)
-        basis[e / (JP - J1)][J1 + e % (JP - J1)] = 0.0;
+    // basis columns past J stay zero for every group
+    double (*bw)[JP] = basis[wave];
+    for (int e = lane; e < 16 * (JP - J1); e += 64) bw[e / (JP - J1)][J1 + e % (JP - J1)] = 0.0;
+    __syncthreads();
     v4d g[NBLK];
     for (int b = 0; b < NBLK; b++) g[b] = (v4d){0.0, 0.0, 0.0, 0.0};
-    const int p0 = wave * 16;                        // this wave's rays in the chunk
-    // the inputs of a chunk's rays (threads 0..63), loaded one chunk ahead
-    // of their use so the loads overlap the previous chunk's work
     struct In { double x, y, t, f; int keep; };
     auto fetch = [&](int64_t b, In& r) {
-        const int64_t i = b + tid;
+        const int64_t i = b + li;
         r.keep = 0;
         r.x = r.y = r.t = r.f = 0.0;
-        if (tid < kZChunk && i < a.n) {
+        if (i < a.n) {
             r.x = a.x[i];
             r.y = a.y[i];
             r.t = a.t[i];
@@ -107,75 +165,60 @@ __global__ __launch_bounds__(kZThreads) void k_zernike_gram(ZArgs a)
             r.keep = a.mask ? (a.mask[i] == 0) : 1;
         }
     };
-    const int64_t stride = (int64_t)gridDim.x * kZChunk;
+    const int64_t stride = (int64_t)gridDim.x * kWaves * 16;
+    int64_t base = ((int64_t)blockIdx.x * kWaves + wave) * 16;
     In cur, nxt;
-    fetch((int64_t)blockIdx.x * kZChunk, cur);
-    for (int64_t base = (int64_t)blockIdx.x * kZChunk; base < a.n; base += stride) {
-        fetch(base + stride, nxt);
+    fetch(base, cur);
+    for (; base < a.n; base += stride) {
+        fetch(base + stride, nxt);                   // one group ahead
         const bool ok = cur.keep != 0;
-        // a chunk whose rays are all masked adds nothing (outside the pupil);
-        // the barrier also orders the previous chunk's reads before these writes
-        if (!__syncthreads_or(ok)) {
+        // a group whose rays are all masked adds nothing (outside the pupil)
+        if (!__any(ok)) {
             cur = nxt;
             continue;
         }
-        if (tid < kZChunk) {
-            double u = 0.0, v = 0.0, w = 0.0;
-            if (ok) {
-                u = cur.x * a.inv_r;
-                v = cur.y * a.inv_r;
-                w = (a.t0 - cur.t) / a.wavelength;       // analysis.py:375, 639
-                if (a.flux) w = w * cur.f;
-            }
-            double up[kZMaxN + 1], vp[kZMaxN + 1];
-            up[0] = ok ? 1.0 : 0.0;
-            vp[0] = 1.0;
-#pragma unroll
-            for (int k = 1; k <= kZMaxN; k++) {
-                up[k] = up[k - 1] * u;
-                vp[k] = vp[k - 1] * v;
-            }
-            // fully unrolled so up[] and vp[] stay in registers
-#pragma unroll
-            for (int d = 0; d <= kZMaxN; d++) {
-                if (d <= a.nmax) {
-#pragma unroll
-                    for (int q = 0; q <= d; q++)
-                        mono[d * (d + 1) / 2 + q][tid] = up[d - q] * vp[q];
-                }
-            }
-            basis[tid][J] = w;
+        double u = 0.0, v = 0.0, w = 0.0;
+        if (ok) {
+            u = cur.x * a.inv_r;
+            v = cur.y * a.inv_r;
+            w = (a.t0 - cur.t) / a.wavelength;       // analysis.py:375, 639
+            if (a.flux) w = w * cur.f;
         }
-        __syncthreads();
-        // 2. basis values of this wave's 16 rays, every 16-column block of C
-        // at once (independent accumulators interleaved)
+        double up[kZMaxN + 1], vp[kZMaxN + 1];
+        up[0] = ok ? 1.0 : 0.0;
+        vp[0] = 1.0;
+#pragma unroll
+        for (int k = 1; k <= kZMaxN; k++) {
+            up[k] = up[k - 1] * u;
+            vp[k] = vp[k - 1] * v;
+        }
+        // the previous group's Gram reads of these rows come first
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        if (lk == 0) bw[li][J] = w;
+        // basis values of the group's 16 rays, every 16-column block of C
         v4d z[NB];
 #pragma unroll
         for (int jb = 0; jb < NB; jb++) z[jb] = (v4d){0.0, 0.0, 0.0, 0.0};
-        for (int ks = 0; ks < ksteps; ks++) {
-            const int m = ks * 4 + lk;
-            const double am = mono[m][p0 + li];
-#pragma unroll
-            for (int jb = 0; jb < NB; jb++)
-                z[jb] = __builtin_amdgcn_mfma_f64_16x16x4f64(am, ct[m][jb * 16 + li], z[jb], 0, 0,
-                                                             0);
-        }
+        basis_steps<NB, KS, 0>(z, up, vp, lk, li, M, ksteps, ct);
 #pragma unroll
         for (int jb = 0; jb < NB; jb++) {
             const int j = jb * 16 + li;
             if (j < J) {
 #pragma unroll
-                for (int e = 0; e < 4; e++) basis[p0 + 4 * e + lk][j] = z[jb][e];
+                for (int e = 0; e < 4; e++) bw[4 * e + lk][j] = z[jb][e];
             }
         }
-        // 3. Gram blocks over the same 16 rays, 4 rays per step; the lane's
-        // value for block row/column x is basis[ray][16 x + li]
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        // Gram blocks over the group, 4 rays per step; the lane's value for
+        // block row/column x is basis[ray][16 x + li]
 #pragma unroll
         for (int ks = 0; ks < 4; ks++) {
-            const int p = p0 + ks * 4 + lk;
+            const int p = ks * 4 + lk;
             double xv[NB];
 #pragma unroll
-            for (int x = 0; x < NB; x++) xv[x] = basis[p][x * 16 + li];
+            for (int x = 0; x < NB; x++) xv[x] = bw[p][x * 16 + li];
             int b = 0;
 #pragma unroll
             for (int rb = 0; rb < NB; rb++)
@@ -186,17 +229,17 @@ __global__ __launch_bounds__(kZThreads) void k_zernike_gram(ZArgs a)
         cur = nxt;
     }
     // the four waves' blocks added in wave order, then the upper triangle out
-    static_assert(NBLK * 256 <= MP * kZChunk, "the wave sums fit in the monomial buffer");
-    double* sum = &mono[0][0];                       // NBLK x 256 doubles
+    static_assert(NBLK * 256 <= kWaves * 16 * JP, "the wave sums fit in the basis buffer");
+    double* sum = &basis[0][0][0];                   // NBLK x 256 doubles
     __syncthreads();
-    for (int w = 0; w < kZThreads / 64; w++) {
-        if (wave == w) {
+    for (int wv = 0; wv < kWaves; wv++) {
+        if (wave == wv) {
 #pragma unroll
             for (int b = 0; b < NBLK; b++)
 #pragma unroll
                 for (int e = 0; e < 4; e++) {
                     double& d = sum[b * 256 + (4 * e + lk) * 16 + li];
-                    d = (w == 0) ? g[b][e] : d + g[b][e];
+                    d = (wv == 0) ? g[b][e] : d + g[b][e];
                 }
         }
         __syncthreads();

@@ -41,6 +41,7 @@ def _numpy_gram(x, y, t, t0, wl, R, jmax, eps=0.0, flux=None, mask=None):
 @pytest.mark.parametrize("n,jmax,eps,use_flux,use_mask",
                          [(1, 1, 0.0, False, False), (63, 4, 0.0, False, True),
                           (64, 22, 0.0, True, False), (65, 37, 0.61, False, True),
+                          (5_001, 55, 0.3, True, False),
                           (100_003, 78, 0.0, True, True), (262_144, 37, 0.0, False, True)])
 def test_gram_matches_numpy(n, jmax, eps, use_flux, use_mask):
     rng = np.random.default_rng(n + jmax)
