@@ -34,7 +34,7 @@ __device__ __forceinline__ void rsplit_ray(
     // traceSplit's filter (optic.py:1262) has already dropped this ray: it
     // stays as it is, and its split part is only marked dropped too
     if (live && (rv.vignetted[i] != 0 || !(rv.flux[i] >= min_flux))) {
-        sp.vignetted[i] = 1;
+        if (sp.vignetted) sp.vignetted[i] = 1;
         return;
     }
     const double* dr = xf.dr;
@@ -145,6 +145,7 @@ __device__ __forceinline__ void rsplit_ray(
     }
     lr = !rvig && rf >= min_flux;
     ls = !svig && sf >= min_flux;
+    if (!sp.flux) return;                           // no split part asked for
     sp.flux[i] = sf;
     sp.vignetted[i] = svig;
     sp.failed[i] = sfail;
@@ -305,10 +306,14 @@ int rsplit_launch(const bt_surface* surface, const double* dr, const double* dro
     if (obsc_len < 0 || (obsc_len > 0 && !obsc))
         return fail(BT_E_SIZE, n + ": bad obscuration program");
     if (niter < 0) return fail(BT_E_SIZE, n + ": negative niter");
-    if (rays.n < 0 || split.n < rays.n)
+    // live mode may ask for no split part: every split column NULL, n = 0
+    const bool no_split = live && split.n == 0 && !split.x && !split.y && !split.z &&
+                          !split.vx && !split.vy && !split.vz && !split.t &&
+                          !split.wavelength && !split.flux && !split.vignetted && !split.failed;
+    if (rays.n < 0 || (!no_split && split.n < rays.n))
         return fail(BT_E_SIZE, n + ": split holds fewer rays");
     if (rays.n == 0) return 0;
-    if (!rays_ok(rays) || !rays_ok(split))
+    if (!rays_ok(rays) || (!no_split && !rays_ok(split)))
         return fail(BT_E_NULL, n + ": incomplete ray columns");
     if (rays.n > (int64_t)0x7fffffff * kBlock)
         return fail(BT_E_SIZE, n + ": too many rays");

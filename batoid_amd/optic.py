@@ -179,11 +179,16 @@ class Interface(Optic):
             base, off = _counts
             live_counts = base[off * _COUNT_SLOTS:(off + 2) * _COUNT_SLOTS]
         bound = getattr(rv, '_flux_bound', None)
-        refracted, reflected = self.rSplit(
-            rv, reverse=reverse, obscuration=self.obscuration,
-            _live_min_flux=minFlux if _filtered else None, _live_counts=live_counts)
+        no_split = False
         if bound is not None:
             ft, fr = self._split_fractions(reverse)
+            # the filter drops every reflected ray: that part is not written
+            no_split = live and not bound * abs(fr) >= minFlux
+        refracted, reflected = self.rSplit(
+            rv, reverse=reverse, obscuration=self.obscuration,
+            _live_min_flux=minFlux if _filtered else None, _live_counts=live_counts,
+            _no_split=no_split)
+        if bound is not None:
             if refracted is not None:
                 refracted._flux_bound = bound * abs(ft)
             if reflected is not None:
@@ -285,7 +290,7 @@ class RefractiveInterface(Interface):
         self.reverseCoating = SimpleCoating(reflectivity=0.0, transmissivity=1.0)
 
     def rSplit(self, rv, reverse=False, obscuration=None, _live_min_flux=None,
-               _live_counts=None):
+               _live_counts=None, _no_split=False):
         """(refracted, reflected); as optic.py:786-797, the forward direction
         uses reverseCoating and the reverse direction forwardCoating."""
         if reverse:
@@ -294,7 +299,7 @@ class RefractiveInterface(Interface):
             m1, m2, coating = self.inMedium, self.outMedium, self.reverseCoating
         return self.surface.rSplit(rv, m1, m2, coating, coordSys=self.coordSys,
                                    obscuration=obscuration, _live_min_flux=_live_min_flux,
-                                   _live_counts=_live_counts)
+                                   _live_counts=_live_counts, _no_split=_no_split)
 
     def _split_fractions(self, reverse):
         coating = self.forwardCoating if reverse else self.reverseCoating
@@ -318,7 +323,7 @@ class Mirror(Interface):
         self.reverseCoating = SimpleCoating(reflectivity=1.0, transmissivity=0.0)
 
     def rSplit(self, rv, reverse=False, obscuration=None, _live_min_flux=None,
-               _live_counts=None):
+               _live_counts=None, _no_split=False):
         from .trace import reflect
         counts = _live_counts[_COUNT_SLOTS:] if _live_counts is not None else None
         return None, reflect(self.surface, rv, self.coordSys, obscuration=obscuration,
@@ -341,11 +346,12 @@ class Detector(Interface):
         self.reverseCoating = None
 
     def rSplit(self, rv, reverse=False, obscuration=None, _live_min_flux=None,
-               _live_counts=None):
+               _live_counts=None, _no_split=False):
         assert not reverse
         return self.surface.rSplit(rv, self.inMedium, self.outMedium, self.forwardCoating,
                                    coordSys=self.coordSys, obscuration=obscuration,
-                                   _live_min_flux=_live_min_flux, _live_counts=_live_counts)
+                                   _live_min_flux=_live_min_flux, _live_counts=_live_counts,
+                                   _no_split=_no_split)
 
     def _split_fractions(self, reverse):
         return self.forwardCoating.transmissivity, self.forwardCoating.reflectivity
@@ -360,7 +366,7 @@ class Baffle(Interface):
         self.reverseCoating = SimpleCoating(reflectivity=0.0, transmissivity=1.0)
 
     def rSplit(self, rv, reverse=False, obscuration=None, _live_min_flux=None,
-               _live_counts=None):
+               _live_counts=None, _no_split=False):
         from .trace import intersect
         counts = _live_counts[:_COUNT_SLOTS] if _live_counts is not None else None
         return intersect(self.surface, rv, self.coordSys, obscuration=obscuration,
@@ -379,7 +385,7 @@ class OPDScreen(Interface):
         raise NotImplementedError("OPDScreen is outside the GPU trace path")
 
     def rSplit(self, rv, reverse=False, obscuration=None, _live_min_flux=None,
-               _live_counts=None):
+               _live_counts=None, _no_split=False):
         raise NotImplementedError("OPDScreen is outside the GPU trace path")
 
 

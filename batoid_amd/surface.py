@@ -93,12 +93,12 @@ class Surface(DeviceCached):
         return reflect(self, rv, coordSys, coating)
 
     def rSplit(self, rv, inMedium, outMedium, coating, coordSys=None, obscuration=None,
-               _live_min_flux=None, _live_counts=None):
+               _live_min_flux=None, _live_counts=None, _no_split=False):
         """Refracted (``rv``, in place) and reflected (new) rays at this
         surface, fluxes split by ``coating`` (surface.py:107-133)."""
         from .trace import rSplit
         return rSplit(self, rv, inMedium, outMedium, coating, coordSys, obscuration,
-                      _live_min_flux, _live_counts)
+                      _live_min_flux, _live_counts, _no_split)
 
     @property
     def _analytic(self):

@@ -138,7 +138,7 @@ def refract(surface, rv, m1, m2, coordSys=None, coating=None, obscuration=None):
 
 
 def rSplit(surface, rv, inMedium, outMedium, coating, coordSys=None, obscuration=None,
-           _live_min_flux=None, _live_counts=None):
+           _live_min_flux=None, _live_counts=None, _no_split=False):
     """Split rays at ``surface`` into refracted (``rv``, in place) and
     reflected (a new RayVector) parts with the coating's flux fractions
     (trace.py:119-136, batoid.cpp:509-674).  ``obscuration``, if given, is
@@ -149,15 +149,20 @@ def rSplit(surface, rv, inMedium, outMedium, coating, coordSys=None, obscuration
     they are skipped and only marked dropped in the reflected part, and a
     reflected ray that filter drops is written only as its flux and flags.
     ``_live_counts``: a device int64[2] the numbers of refracted and
-    reflected rays passing that filter are added to."""
+    reflected rays passing that filter are added to.  ``_no_split`` (with
+    ``_live_min_flux``): traceSplit knows that its filter drops every
+    reflected ray; no reflected part is written and None is returned in
+    its place."""
     from .rayVector import RayVector
     if coordSys is None:
         coordSys = rv.coordSys
     ct = CoordTransform(rv.coordSys, coordSys)
-    split = RayVector._empty(len(rv), coordSys)
+    no_split = _no_split and _live_min_flux is not None
+    split = None if no_split else RayVector._empty(len(rv), coordSys)
     optr, olen = _obsc_args(obscuration)
     args = (surface._dev.ptr(), _lib.dvec(ct.dr), _lib.dvec(ct.drot.ravel()),
-            outMedium._dev.ptr(), _coating_ptr(coating), optr, olen, rv._rays, split._rays)
+            outMedium._dev.ptr(), _coating_ptr(coating), optr, olen, rv._rays,
+            _lib.BtRays() if no_split else split._rays)
     if _live_min_flux is None:
         _lib.call("bt_rsplit", *args, _niter(), _device.stream())
     else:

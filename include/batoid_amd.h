@@ -395,7 +395,10 @@ int bt_rsplit(const bt_surface* surface, const double* dr, const double* drot,
  * written); every other ray is split exactly as by bt_rsplit, except that
  * a split (reflected) ray the same filter drops is written only as its
  * flux, vignetted and failed.  Filtering the two outputs afterwards gives
- * the rays bt_rsplit on the filtered set and the same filter would give.  `kinds`: BT_PLAN_ANALYTIC when the
+ * the rays bt_rsplit on the filtered set and the same filter would give.
+ * A `split` with every column NULL and n = 0 asks for no split part: only
+ * the refracted part is written (traceSplit's use when it already knows
+ * that the filter drops every reflected ray).  `kinds`: BT_PLAN_ANALYTIC when the
  * surface record is a Plane, Sphere, Paraboloid, Quadric, Asphere or Tilted
  * (a kernel without the other kinds' code; under it any other kind fails
  * its rays), else 0.  `counts` (a device int64[2 * 64], or NULL) is

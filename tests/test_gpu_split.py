@@ -449,3 +449,32 @@ def test_traceSplit_flux_bound_preconditions():
     item = next(i for i in optic.itemDict.values() if isinstance(i, bt.RefractiveInterface))
     item.reverseCoating = bt.SimpleCoating(1.0, 0.0)
     assert optic._flux_bound(rays, 1e-3) is None
+
+
+def test_rsplit_live_without_split_part():
+    """bt_rsplit_live asked for no split part (every split column NULL,
+    n = 0): the refracted part is bit-identical to the call with one; a
+    split with only some columns NULL is still refused."""
+    from batoid_amd import trace as T
+    from batoid_amd import _device
+    surf = dict(surfaces())["asphere_l2"]
+    med = media()[2][1]
+    coat = bt.SimpleCoating(0.02, 0.98)
+    rng = np.random.default_rng(41)
+    base = HostRays(**random_rays(N, seed=43, upward=0.3, spread=0.3))
+    base.cols["flux"] = rng.uniform(0.0, 3e-2, N)
+    minFlux = 1e-3
+    with_r, with_s = T.rSplit(surf, _gpu(base), bt.ConstMedium(1.0), med, coat,
+                              _live_min_flux=minFlux)
+    none_r, none_s = T.rSplit(surf, _gpu(base), bt.ConstMedium(1.0), med, coat,
+                              _live_min_flux=minFlux, _no_split=True)
+    assert none_s is None and with_s is not None
+    assert_cols_identical(_host(none_r), _host(with_r), what="refracted part")
+    rv = _gpu(base)
+    half = _lib.BtRays()
+    half.flux = ctypes.c_void_p(rv.torch("flux").data_ptr())
+    ct = bt.CoordTransform(rv.coordSys, rv.coordSys)
+    with pytest.raises(RuntimeError):
+        _lib.call("bt_rsplit_live", surf._dev.ptr(), _lib.dvec(ct.dr),
+                  _lib.dvec(ct.drot.ravel()), med._dev.ptr(), ctypes.byref(coat._record), None,
+                  0, rv._rays, half, minFlux, 5, 0, None, _device.stream())
