@@ -150,9 +150,9 @@ __device__ __forceinline__ void rsplit_ray(
     sp.vignetted[i] = svig;
     sp.failed[i] = sfail;
     // live mode: a split ray the filter drops (vignetted, or its flux below
-    // min_flux -- all of them once a ray has been reflected twice by 2 %
-    // glass at min_flux 1e-4) is never copied out or traced further, and
-    // every later pass reads only its flux and flags
+    // min_flux -- every one once the ray's flux times the reflectivity is)
+    // is never copied out or traced further, and every later pass reads
+    // only its flux and flags
     if (live && !ls) return;
     sp.x[i] = sx; sp.y[i] = sy; sp.z[i] = sz;
     sp.vx[i] = svx; sp.vy[i] = svy; sp.vz[i] = svz;
